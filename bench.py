@@ -1,0 +1,192 @@
+"""Benchmark of the BGK3D_HIT hot path on one MI355X.
+
+Workload (one "step" = one fused stream-collide sweep of the lattice):
+512^3 triply periodic D3Q19 BGK, seeded HIT initial field (k0 = 4,
+u_rms = 0.02, seed 1234), omega = 1.95, with the energy/dissipation
+reductions (diagno + dissipation) every 50 steps inside the timed region.
+The headline `value` is the FP32 run (storage and compute FP32); the FP64
+and reduced-storage (FP16 / BF16 shifted f_i - w_i, FP32 collide) runs of
+the same workload are reported under `variants`.
+
+`roofline` is measured live: HIP events around every collide launch on the
+stream it runs on give the average launch duration; the algorithmic bytes
+per launch are 19 loads + 19 stores per site (38 * s * 512^3).  `traffic`
+is the HBM bytes per launch measured by rocprofv3 FETCH_SIZE / WRITE_SIZE
+runs of this script (profiles/collide_traffic.json), null when absent.
+
+`cpu_baseline` times the CPU oracle (plain C/OpenMP restatement of the
+reference Fortran) on a bounded sample on the host cores.
+
+Prints one JSON line on stdout; progress goes to stderr.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "MLUPS + % HBM roofline, 512^3 D3Q19 (FP64/FP32/FP16-store); adjoint/fwd time"
+HBM_PEAK_GBS = 8000.0
+BYTES = {"f64": 8, "f32": 4, "f32mix": 4, "f16": 2, "bf16": 2}
+COMPUTE = {"f64": "f64", "f32": "f32", "f32mix": "f64", "f16": "f32", "bf16": "f32"}
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def run_gpu(kind, n, steps, warmup, diag_every, omega=1.95):
+    import bgk3d_hit_amd as bgk
+
+    nu = bgk.svisc_from_omega(omega)
+    lat = bgk.Lattice(n, n, n, omega=omega, storage=kind, timing=True)
+    try:
+        lat.init_hit(4.0, 0.02, 1234)
+        d0 = lat.diagno()
+        lat.step(warmup)
+        lat.synchronize()
+        lat.timing(reset=True)
+        diag = []
+        t0 = time.perf_counter()
+        done = 0
+        while done < steps:
+            k = min(diag_every, steps - done)
+            lat.step(k)
+            done += k
+            if done % diag_every == 0:
+                diag.append((done + warmup, *lat.diagno(), *lat.dissipation(nu)))
+        lat.synchronize()
+        wall = time.perf_counter() - t0
+        t = lat.timing(reset=True)
+    finally:
+        lat.close()
+    nsite = n ** 3
+    avg_ms = t["collide_ms"] / max(t["collide_launches"], 1)
+    algo_bytes = 38 * BYTES[kind] * nsite
+    achieved = algo_bytes / (avg_ms * 1e-3) / 1e9
+    return {
+        "kind": kind,
+        "mlups": nsite * steps / wall / 1e6,
+        "ms_per_step": wall * 1e3 / steps,
+        "collide_avg_ms": avg_ms,
+        "collide_launches": t["collide_launches"],
+        "reduce_ms_total": t["reduce_ms"],
+        "achieved_gbs": achieved,
+        "frac": achieved / HBM_PEAK_GBS,
+        "algo_bytes_per_launch": algo_bytes,
+        "ke0": d0[4],
+        "diag_last": diag[-1] if diag else None,
+    }
+
+
+def cpu_baseline(min_seconds=10.0, max_steps=8):
+    import numpy as np
+
+    import bgk3d_hit_amd as bgk
+    import oracle
+
+    n = 256
+    u = bgk.hit_velocity(n, n, n, 4.0, 0.02, 1234)
+    o = oracle.OracleLattice(n, n, n, omega=1.95, kind="f32")
+    o.init_equilibrium(u)
+    del u
+    o.step(1)  # first touch of every page
+    steps, t0 = 0, time.perf_counter()
+    while steps < max_steps and (time.perf_counter() - t0) < min_seconds:
+        o.step(1)
+        steps += 1
+    wall = time.perf_counter() - t0
+    o.close()
+    return {"value": round(n ** 3 * steps / wall / 1e6, 2), "unit": "MLUPS",
+            "cores": oracle.num_threads(), "kind": "port",
+            "sample": f"CPU oracle (C/OpenMP restatement of SRC/col_MC.F90 + "
+                      f"SRC/bcond.comm.step3.F90), FP32, 256^3 HIT sample of the same "
+                      f"workload, {steps} steps in {wall:.1f} s"}
+
+
+def load_traffic(kind):
+    p = os.path.join(ROOT, "profiles", "collide_traffic.json")
+    try:
+        with open(p) as f:
+            d = json.load(f)
+        return d.get(kind, {}).get("hbm_bytes_per_launch")
+    except (OSError, ValueError):
+        return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--size", type=int, default=512)
+    ap.add_argument("--diag-every", type=int, default=50)
+    ap.add_argument("--variants", default="f64,f16,bf16",
+                    help="extra storage kinds to measure beside f32 ('' for none)")
+    ap.add_argument("--no-cpu", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only (asked for --gpus {a.gpus}); "
+              "the one-GPU throughput is the whole-job value", file=sys.stderr)
+        sys.exit(2)
+    import bgk3d_hit_amd as bgk
+
+    if bgk.device_count() < 1:
+        log("bench.py: no GPU visible")
+        sys.exit(1)
+    log(f"[bench] f32 {a.size}^3 warmup {a.warmup} steps {a.steps}")
+    main_run = run_gpu("f32", a.size, a.steps, a.warmup, a.diag_every)
+    log(f"[bench] f32: {main_run['mlups']:.0f} MLUPS, collide {main_run['collide_avg_ms']:.3f} ms"
+        f" = {main_run['achieved_gbs']:.0f} GB/s")
+    variants = {}
+    for kind in [k for k in a.variants.split(",") if k]:
+        log(f"[bench] {kind} {a.size}^3")
+        r = run_gpu(kind, a.size, a.steps, a.warmup, a.diag_every)
+        log(f"[bench] {kind}: {r['mlups']:.0f} MLUPS, {r['achieved_gbs']:.0f} GB/s")
+        variants[kind] = {
+            "mlups": round(r["mlups"], 1), "ms_per_step": round(r["ms_per_step"], 4),
+            "collide_avg_ms": round(r["collide_avg_ms"], 4),
+            "achieved_gbs": round(r["achieved_gbs"], 1), "frac": round(r["frac"], 4),
+            "storage_bytes": BYTES[kind], "compute": COMPUTE[kind],
+            "traffic": load_traffic(kind),
+            "speedup_vs_f32": round(r["mlups"] / main_run["mlups"], 3)}
+    cpu = None
+    if not a.no_cpu:
+        log("[bench] cpu baseline")
+        cpu = cpu_baseline()
+        log(f"[bench] cpu: {cpu['value']} MLUPS on {cpu['cores']} threads")
+    out = {
+        "metric": METRIC,
+        "value": round(main_run["mlups"], 1),
+        "unit": "MLUPS",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(main_run["ms_per_step"], 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic: seeded HIT velocity (k0=4, u_rms=0.02, seed 1234), rho=1 "
+                "equilibrium populations",
+        "config": {"workload": f"{a.size}^3 periodic D3Q19 BGK HIT, omega=1.95, FP32 storage "
+                               f"and collide, diagno+dissipation every {a.diag_every} steps",
+                   "lattice": [a.size] * 3, "omega": 1.95, "diag_every": a.diag_every,
+                   "layout": "SoA, halo-free, periodic wrap in index math"},
+        "roofline": {"bound": "hbm", "achieved": round(main_run["achieved_gbs"], 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(main_run["frac"], 4), "traffic": load_traffic("f32"),
+                     "kernel": "k_collide<float,float>",
+                     "avg_launch_ms": round(main_run["collide_avg_ms"], 4),
+                     "algo_bytes_per_launch": main_run["algo_bytes_per_launch"]},
+        "cpu_baseline": cpu,
+        "variants": variants,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,686 @@
+// bgk_capi.hip -- the extern "C" entry points of include/bgk3d_hit.h.
+//
+// The context replaces the reference's module globals (SRC/storage.F90) and
+// its OpenACC data region (SRC/bgk3d_HIT.F90:111-115): both population copies
+// stay resident in HBM for the context's lifetime and the a/b pointer swap
+// (SRC/col_MC.F90:365-421) is the index `cur`.
+#include <math.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <string>
+#include <vector>
+
+#include "bgk_internal.h"
+
+namespace bgk {
+
+static thread_local std::string g_err;
+
+void set_error(const std::string &msg) { g_err = msg; }
+int fail(int code, const std::string &msg) {
+    g_err = msg;
+    return code;
+}
+int hip_fail(hipError_t e, const char *what) {
+    g_err = std::string("HIP error ") + hipGetErrorName(e) + " (" + hipGetErrorString(e) +
+            ") in " + what;
+    return e == hipErrorOutOfMemory ? BGK_ENOMEM : BGK_EHIP;
+}
+
+size_t storage_bytes(int storage) {
+    switch (storage) {
+    case BGK_STORE_F64: return 8;
+    case BGK_STORE_F32:
+    case BGK_STORE_F32_MIXED: return 4;
+    case BGK_STORE_F16:
+    case BGK_STORE_BF16: return 2;
+    default: return 0;
+    }
+}
+
+// ---------------------------------------------------------------- timing
+static hipError_t next_event(bgk_ctx *c, hipEvent_t *ev) {
+    Timing &T = c->timing;
+    if (T.used == T.pool.size()) {
+        hipEvent_t e;
+        hipError_t r = hipEventCreate(&e);
+        if (r != hipSuccess) return r;
+        T.pool.push_back(e);
+    }
+    *ev = T.pool[T.used++];
+    return hipSuccess;
+}
+
+static hipError_t harvest(bgk_ctx *c) {
+    Timing &T = c->timing;
+    if (T.recs.empty()) return hipSuccess;
+    hipError_t r = hipEventSynchronize(T.recs.back().b);
+    if (r != hipSuccess) return r;
+    for (auto &rec : T.recs) {
+        r = hipEventSynchronize(rec.b);
+        if (r != hipSuccess) return r;
+        float ms = 0.f;
+        r = hipEventElapsedTime(&ms, rec.a, rec.b);
+        if (r != hipSuccess) return r;
+        T.ms[rec.cls] += ms;
+        T.launches[rec.cls] += 1;
+    }
+    T.recs.clear();
+    T.used = 0;
+    return hipSuccess;
+}
+
+hipError_t timing_begin(bgk_ctx *c, int cls, hipStream_t s, hipEvent_t *out) {
+    *out = nullptr;
+    if (!c->timing.on) return hipSuccess;
+    if (c->timing.recs.size() > 8192) {
+        hipError_t r = harvest(c);
+        if (r != hipSuccess) return r;
+    }
+    hipError_t r = next_event(c, out);
+    if (r != hipSuccess) return r;
+    (void)cls;
+    return hipEventRecord(*out, s);
+}
+
+hipError_t timing_end(bgk_ctx *c, int cls, hipStream_t s, hipEvent_t begin) {
+    if (!c->timing.on || !begin) return hipSuccess;
+    hipEvent_t e;
+    hipError_t r = next_event(c, &e);
+    if (r != hipSuccess) return r;
+    r = hipEventRecord(e, s);
+    if (r != hipSuccess) return r;
+    c->timing.recs.push_back({cls, begin, e});
+    return hipSuccess;
+}
+
+// ------------------------------------------------------------- region boxes
+static Box full_box(const Lattice &L) { return Box{0, L.nx, 0, L.ny, 0, L.nz}; }
+
+// bulk / boundary split of col_MC_masked.  Periodic layout: the reference
+// mask (SRC/init.F90:44-62): bulk = sites at least `border` away from every
+// face.  Slab layout: bulk = planes that do not read a ghost plane.
+static void region_boxes(const bgk_ctx *c, int opt, std::vector<Box> &out) {
+    const Lattice &L = c->lat;
+    out.clear();
+    if (L.layout == BGK_LAYOUT_SLAB_Z) {
+        if (opt == 0) {
+            if (L.nz > 2) out.push_back(Box{0, L.nx, 0, L.ny, 1, L.nz - 1});
+        } else {
+            out.push_back(Box{0, L.nx, 0, L.ny, 0, 1});
+            if (L.nz > 1) out.push_back(Box{0, L.nx, 0, L.ny, L.nz - 1, L.nz});
+        }
+        return;
+    }
+    const int b = c->border;
+    const bool has_bulk = (2 * b < L.nx) && (2 * b < L.ny) && (2 * b < L.nz);
+    if (!has_bulk) {
+        if (opt == 1) out.push_back(full_box(L));
+        return;
+    }
+    if (opt == 0) {
+        out.push_back(Box{b, L.nx - b, b, L.ny - b, b, L.nz - b});
+        return;
+    }
+    if (b == 0) return;
+    out.push_back(Box{0, L.nx, 0, L.ny, 0, b});
+    out.push_back(Box{0, L.nx, 0, L.ny, L.nz - b, L.nz});
+    out.push_back(Box{0, L.nx, 0, b, b, L.nz - b});
+    out.push_back(Box{0, L.nx, L.ny - b, L.ny, b, L.nz - b});
+    out.push_back(Box{0, b, b, L.ny - b, b, L.nz - b});
+    out.push_back(Box{L.nx - b, L.nx, b, L.ny - b, b, L.nz - b});
+}
+
+static int collide_boxes(bgk_ctx *c, const std::vector<Box> &boxes, hipStream_t s) {
+    for (const Box &b : boxes) {
+        hipEvent_t ev;
+        BGK_HIP(timing_begin(c, TCLS_COLLIDE, s, &ev));
+        BGK_HIP(launch_collide(c->lat, c->cur, b, c->omega, s));
+        BGK_HIP(timing_end(c, TCLS_COLLIDE, s, ev));
+    }
+    return BGK_OK;
+}
+
+// slab halo exchange of copy `cur` on stream s: pack the 5 crossing
+// populations of both z faces, deliver the messages (periodic self-exchange:
+// this rank is its own up and down neighbour), unpack into the ghost planes
+static int halo_exchange(bgk_ctx *c, hipStream_t s) {
+    hipEvent_t ev;
+    BGK_HIP(timing_begin(c, TCLS_HALO, s, &ev));
+    BGK_HIP(launch_pack_z(c->lat, c->cur, c->halo_send, s));
+    BGK_HIP(hipMemcpyAsync(c->halo_recv, c->halo_send, c->halo_bytes, hipMemcpyDeviceToDevice, s));
+    BGK_HIP(launch_unpack_z(c->lat, c->cur, c->halo_recv, s));
+    BGK_HIP(timing_end(c, TCLS_HALO, s, ev));
+    return BGK_OK;
+}
+
+static int record_faces(bgk_ctx *c, hipStream_t s) {
+    hipEvent_t ev;
+    BGK_HIP(timing_begin(c, TCLS_HALO, s, &ev));
+    BGK_HIP(launch_pack_faces(c->lat, c->cur, c->gcache[c->cur], s));
+    BGK_HIP(timing_end(c, TCLS_HALO, s, ev));
+    return BGK_OK;
+}
+
+int step_periodic(bgk_ctx *c, int nsteps) {
+    std::vector<Box> all{full_box(c->lat)};
+    for (int t = 0; t < nsteps; t++) {
+        if (c->flags & BGK_FLAG_REFCOMPAT_HALOS) {
+            int r = record_faces(c, c->s0);
+            if (r) return r;
+        }
+        int r = collide_boxes(c, all, c->s0);
+        if (r) return r;
+        c->cur ^= 1;
+        c->steps++;
+    }
+    return BGK_OK;
+}
+
+// Slab time step with overlap (the structure of SRC/bcond.comm.step9.F90:
+// pack, exchange, bulk collide overlapping the exchange, unpack, border
+// collide).  Stream s0 runs the bulk planes 1..nz-2, stream s1 the halo path
+// and the two boundary planes.  Cross-stream dependencies per step t:
+//   s1 boundary(t) reads planes 1 and nz-2 written by s0 bulk(t-1)
+//   s0 bulk(t) reads planes 0 and nz-1 written by s1 boundary(t-1) and
+//   overwrites the buffer s1 boundary(t-1) read.
+static int step_slab(bgk_ctx *c, int nsteps) {
+    std::vector<Box> bulk, border;
+    region_boxes(c, 0, bulk);
+    region_boxes(c, 1, border);
+    // both streams start after all previously queued work of the context
+    BGK_HIP(hipEventRecord(c->ev0, c->s0));
+    BGK_HIP(hipStreamWaitEvent(c->s1, c->ev0, 0));
+    BGK_HIP(hipEventRecord(c->ev1, c->s1));
+    for (int t = 0; t < nsteps; t++) {
+        // s1: halo path does not depend on s0's previous step
+        int r = halo_exchange(c, c->s1);
+        if (r) return r;
+        BGK_HIP(hipStreamWaitEvent(c->s1, c->ev0, 0));  // bulk(t-1) done
+        // s0: bulk(t) after boundary(t-1)
+        BGK_HIP(hipStreamWaitEvent(c->s0, c->ev1, 0));
+        r = collide_boxes(c, bulk, c->s0);
+        if (r) return r;
+        r = collide_boxes(c, border, c->s1);
+        if (r) return r;
+        BGK_HIP(hipEventRecord(c->ev0, c->s0));
+        BGK_HIP(hipEventRecord(c->ev1, c->s1));
+        c->cur ^= 1;
+        c->steps++;
+    }
+    // join: later work on s0 sees the boundary planes
+    BGK_HIP(hipStreamWaitEvent(c->s0, c->ev1, 0));
+    return BGK_OK;
+}
+
+int reduce_ke(bgk_ctx *c, int copy, double *J) {
+    BGK_HIP(launch_diagno(c->lat, copy, c->red_partials, c->red_blocks, c->red_out, c->s0));
+    BGK_HIP(hipMemcpyAsync(c->red_host, c->red_out, 8 * sizeof(double), hipMemcpyDeviceToHost,
+                           c->s0));
+    BGK_HIP(hipStreamSynchronize(c->s0));
+    *J = c->red_host[4];
+    if (!std::isfinite(*J)) return fail(BGK_ENONFINITE, "kinetic energy is not finite");
+    return BGK_OK;
+}
+
+static int check_ctx(bgk_ctx *c) {
+    if (!c) return fail(BGK_EINVAL, "null context");
+    hipError_t e = hipSetDevice(c->device);
+    if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+    return BGK_OK;
+}
+
+static const long long CHUNK = 1LL << 23;  // sites per host transfer chunk
+
+}  // namespace bgk
+
+using namespace bgk;
+
+extern "C" {
+
+int bgk_version(void) { return 100; }
+
+int bgk_last_error(char *buf, int len) {
+    if (!buf || len <= 0) return BGK_EINVAL;
+    snprintf(buf, (size_t)len, "%s", g_err.c_str());
+    return BGK_OK;
+}
+
+int bgk_device_count(int *count) {
+    if (!count) return fail(BGK_EINVAL, "null count");
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess) n = 0;
+    *count = n;
+    return BGK_OK;
+}
+
+int bgk_create(int lx, int ly, int lz, int storage, int layout, int flags, double omega,
+               bgk_ctx **out) {
+    if (!out) return fail(BGK_EINVAL, "null output pointer");
+    *out = nullptr;
+    if (lx < 1 || ly < 1 || lz < 1)
+        return fail(BGK_EINVAL, "lattice sizes must be positive");
+    if (storage_bytes(storage) == 0) return fail(BGK_EINVAL, "unknown storage kind");
+    if (layout != BGK_LAYOUT_PERIODIC && layout != BGK_LAYOUT_SLAB_Z)
+        return fail(BGK_EINVAL, "unknown layout");
+    if (layout == BGK_LAYOUT_SLAB_Z && lz < 2)
+        return fail(BGK_EINVAL, "slab layout needs at least 2 planes");
+    if (layout == BGK_LAYOUT_SLAB_Z && (flags & BGK_FLAG_REFCOMPAT_HALOS))
+        return fail(BGK_EINVAL, "refcompat halo images apply to the periodic layout");
+    if (!(omega >= 0.0 && omega < 2.0))
+        return fail(BGK_EINVAL, "omega must lie in [0, 2)");
+    const int nzs = layout == BGK_LAYOUT_SLAB_Z ? lz + 2 : lz;
+    const long long plane = (long long)lx * ly * nzs;
+    if (plane >= (1LL << 31))
+        return fail(BGK_EINVAL, "more than 2^31 stored sites per population");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+        return fail(BGK_ENODEV, "no GPU visible to the HIP runtime");
+
+    bgk_ctx *c = new bgk_ctx();
+    (void)hipGetDevice(&c->device);
+    Lattice &L = c->lat;
+    L.nx = lx;
+    L.ny = ly;
+    L.nz = lz;
+    L.layout = layout;
+    L.storage = storage;
+    L.zoff = layout == BGK_LAYOUT_SLAB_Z ? 1 : 0;
+    L.nzs = nzs;
+    L.shifted = (flags & BGK_FLAG_NOSHIFT) ? 0 : 1;
+    L.nsite = (long long)lx * ly * lz;
+    L.plane_elems = plane;
+    c->flags = flags;
+    c->omega = omega;
+    c->timing.on = (flags & BGK_FLAG_TIMING) != 0;
+
+    const size_t es = storage_bytes(storage);
+    const size_t pbytes = ((size_t)plane * es + 255) & ~(size_t)255;
+    int rc = BGK_OK;
+    auto bail = [&](int code) {
+        bgk_destroy(c);
+        return code;
+    };
+    hipError_t e = hipMalloc(&c->pop_alloc, pbytes * 37);
+    if (e != hipSuccess) return bail(hip_fail(e, "hipMalloc(populations)"));
+    c->bytes += pbytes * 37;
+    char *base = (char *)c->pop_alloc;
+    for (int q = 0; q < 18; q++) {
+        L.pop[0][q] = base + pbytes * q;
+        L.pop[1][q] = base + pbytes * (18 + q);
+    }
+    L.pop[0][18] = L.pop[1][18] = base + pbytes * 36;
+
+    if ((e = hipStreamCreateWithFlags(&c->s0, hipStreamNonBlocking)) != hipSuccess)
+        return bail(hip_fail(e, "hipStreamCreate"));
+    if ((e = hipStreamCreateWithFlags(&c->s1, hipStreamNonBlocking)) != hipSuccess)
+        return bail(hip_fail(e, "hipStreamCreate"));
+    if ((e = hipEventCreateWithFlags(&c->ev0, hipEventDisableTiming)) != hipSuccess)
+        return bail(hip_fail(e, "hipEventCreate"));
+    if ((e = hipEventCreateWithFlags(&c->ev1, hipEventDisableTiming)) != hipSuccess)
+        return bail(hip_fail(e, "hipEventCreate"));
+    if ((e = hipMemsetAsync(c->pop_alloc, 0, pbytes * 37, c->s0)) != hipSuccess)
+        return bail(hip_fail(e, "hipMemsetAsync(populations)"));
+
+    if (layout == BGK_LAYOUT_SLAB_Z) {
+        c->halo_bytes = (size_t)2 * 5 * lx * ly * es;
+        if ((e = hipMalloc(&c->halo_send, c->halo_bytes)) != hipSuccess)
+            return bail(hip_fail(e, "hipMalloc(halo)"));
+        if ((e = hipMalloc(&c->halo_recv, c->halo_bytes)) != hipSuccess)
+            return bail(hip_fail(e, "hipMalloc(halo)"));
+        c->bytes += 2 * c->halo_bytes;
+    }
+    if (flags & BGK_FLAG_REFCOMPAT_HALOS) {
+        c->gcache_bytes =
+            (size_t)19 * ((size_t)ly * lz + (size_t)lx * lz + (size_t)lx * ly) * es;
+        for (int k = 0; k < 2; k++) {
+            if ((e = hipMalloc(&c->gcache[k], c->gcache_bytes)) != hipSuccess)
+                return bail(hip_fail(e, "hipMalloc(halo images)"));
+            if ((e = hipMemsetAsync(c->gcache[k], 0, c->gcache_bytes, c->s0)) != hipSuccess)
+                return bail(hip_fail(e, "hipMemsetAsync(halo images)"));
+        }
+        c->bytes += 2 * c->gcache_bytes;
+    }
+    long long nb = (L.nsite + 255) / 256;
+    c->red_blocks = (int)std::min<long long>(std::max<long long>(nb, 1), 2048);
+    if ((e = hipMalloc(&c->red_partials, sizeof(double) * 8 * c->red_blocks)) != hipSuccess)
+        return bail(hip_fail(e, "hipMalloc(reduction)"));
+    if ((e = hipMalloc(&c->red_out, sizeof(double) * 8)) != hipSuccess)
+        return bail(hip_fail(e, "hipMalloc(reduction)"));
+    if ((e = hipHostMalloc(&c->red_host, sizeof(double) * 8, hipHostMallocDefault)) !=
+        hipSuccess)
+        return bail(hip_fail(e, "hipHostMalloc(reduction)"));
+    const long long chunk = std::min(L.nsite, CHUNK);
+    if ((e = hipMalloc(&c->staging, sizeof(double) * 3 * chunk)) != hipSuccess)
+        return bail(hip_fail(e, "hipMalloc(staging)"));
+    c->bytes += sizeof(double) * (3 * chunk + 8 * c->red_blocks + 8);
+    if ((e = hipStreamSynchronize(c->s0)) != hipSuccess)
+        return bail(hip_fail(e, "hipStreamSynchronize"));
+    (void)rc;
+    *out = c;
+    return BGK_OK;
+}
+
+int bgk_destroy(bgk_ctx *c) {
+    if (!c) return BGK_OK;
+    (void)hipSetDevice(c->device);
+    if (c->s0) (void)hipStreamSynchronize(c->s0);
+    if (c->s1) (void)hipStreamSynchronize(c->s1);
+    for (hipEvent_t e : c->timing.pool) (void)hipEventDestroy(e);
+    if (c->ev0) (void)hipEventDestroy(c->ev0);
+    if (c->ev1) (void)hipEventDestroy(c->ev1);
+    if (c->s0) (void)hipStreamDestroy(c->s0);
+    if (c->s1) (void)hipStreamDestroy(c->s1);
+    (void)hipFree(c->pop_alloc);
+    (void)hipFree(c->halo_send);
+    (void)hipFree(c->halo_recv);
+    (void)hipFree(c->gcache[0]);
+    (void)hipFree(c->gcache[1]);
+    (void)hipFree(c->red_partials);
+    (void)hipFree(c->red_out);
+    if (c->red_host) (void)hipHostFree(c->red_host);
+    (void)hipFree(c->staging);
+    delete c;
+    return BGK_OK;
+}
+
+int bgk_set_omega(bgk_ctx *c, double omega) {
+    if (!c) return fail(BGK_EINVAL, "null context");
+    if (!(omega >= 0.0 && omega < 2.0)) return fail(BGK_EINVAL, "omega must lie in [0, 2)");
+    c->omega = omega;
+    return BGK_OK;
+}
+
+int bgk_set_border(bgk_ctx *c, int border) {
+    if (!c) return fail(BGK_EINVAL, "null context");
+    if (border < 0) return fail(BGK_EINVAL, "border must be >= 0");
+    c->border = border;
+    return BGK_OK;
+}
+
+int bgk_info(bgk_ctx *c, int64_t info[8]) {
+    if (!c || !info) return fail(BGK_EINVAL, "null argument");
+    info[0] = c->lat.nx;
+    info[1] = c->lat.ny;
+    info[2] = c->lat.nz;
+    info[3] = c->lat.storage;
+    info[4] = c->lat.layout;
+    info[5] = c->flags;
+    info[6] = (int64_t)c->bytes;
+    info[7] = c->steps;
+    return BGK_OK;
+}
+
+int bgk_set_populations(bgk_ctx *c, const double *f) {
+    int r = check_ctx(c);
+    if (r) return r;
+    if (!f) return fail(BGK_EINVAL, "null population array");
+    const Lattice &L = c->lat;
+    const long long chunk = std::min(L.nsite, CHUNK);
+    for (int q = 0; q < 19; q++)
+        for (long long i0 = 0; i0 < L.nsite; i0 += chunk) {
+            const long long n = std::min(chunk, L.nsite - i0);
+            BGK_HIP(hipMemcpyAsync(c->staging, f + (size_t)q * L.nsite + i0, sizeof(double) * n,
+                                   hipMemcpyHostToDevice, c->s0));
+            BGK_HIP(launch_plane_from_f64(L, c->cur, q, c->staging, i0, n, c->s0));
+        }
+    BGK_HIP(hipStreamSynchronize(c->s0));
+    return BGK_OK;
+}
+
+int bgk_get_populations(bgk_ctx *c, double *f) {
+    int r = check_ctx(c);
+    if (r) return r;
+    if (!f) return fail(BGK_EINVAL, "null population array");
+    const Lattice &L = c->lat;
+    const long long chunk = std::min(L.nsite, CHUNK);
+    for (int q = 0; q < 19; q++)
+        for (long long i0 = 0; i0 < L.nsite; i0 += chunk) {
+            const long long n = std::min(chunk, L.nsite - i0);
+            BGK_HIP(launch_plane_to_f64(L, c->cur, q, c->staging, i0, n, c->s0));
+            BGK_HIP(hipMemcpyAsync(f + (size_t)q * L.nsite + i0, c->staging, sizeof(double) * n,
+                                   hipMemcpyDeviceToHost, c->s0));
+            BGK_HIP(hipStreamSynchronize(c->s0));
+        }
+    return BGK_OK;
+}
+
+int bgk_init_equilibrium(bgk_ctx *c, const double *u) {
+    int r = check_ctx(c);
+    if (r) return r;
+    if (!u) return fail(BGK_EINVAL, "null velocity array");
+    const Lattice &L = c->lat;
+    const long long chunk = std::min(L.nsite, CHUNK);
+    for (long long i0 = 0; i0 < L.nsite; i0 += chunk) {
+        const long long n = std::min(chunk, L.nsite - i0);
+        for (int d = 0; d < 3; d++)
+            BGK_HIP(hipMemcpyAsync(c->staging + (size_t)d * n, u + (size_t)d * L.nsite + i0,
+                                   sizeof(double) * n, hipMemcpyHostToDevice, c->s0));
+        BGK_HIP(launch_init_equilibrium(L, c->cur, c->staging, n, i0, n, c->s0));
+        BGK_HIP(hipStreamSynchronize(c->s0));
+    }
+    return BGK_OK;
+}
+
+int bgk_hit_velocity(int lx, int ly, int lz, double k0, double urms, uint64_t seed, double *u) {
+    if (!u) return fail(BGK_EINVAL, "null velocity array");
+    HitSpectrum sp;
+    int r = hit_spectrum(lx, ly, lz, k0, urms, seed, sp);
+    if (r) return r;
+    hit_synthesize_host(lx, ly, lz, sp, u);
+    return BGK_OK;
+}
+
+int bgk_init_hit(bgk_ctx *c, double k0, double urms, uint64_t seed) {
+    int r = check_ctx(c);
+    if (r) return r;
+    const Lattice &L = c->lat;
+    HitSpectrum sp;
+    r = hit_spectrum(L.nx, L.ny, L.nz, k0, urms, seed, sp);
+    if (r) return r;
+    double *u = nullptr;
+    hipError_t e = hipMalloc(&u, sizeof(double) * 3 * L.nsite);
+    if (e != hipSuccess) return hip_fail(e, "hipMalloc(velocity field)");
+    r = hit_synthesize_device(c, sp, u);
+    if (r == BGK_OK) {
+        e = launch_init_equilibrium(L, c->cur, u, L.nsite, 0, L.nsite, c->s0);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->s0);
+        if (e != hipSuccess) r = hip_fail(e, "init_equilibrium");
+    }
+    (void)hipFree(u);
+    return r;
+}
+
+int bgk_boundaries(bgk_ctx *c) {
+    int r = check_ctx(c);
+    if (r) return r;
+    if (c->lat.layout == BGK_LAYOUT_SLAB_Z) return halo_exchange(c, c->s0);
+    if (c->flags & BGK_FLAG_REFCOMPAT_HALOS) return record_faces(c, c->s0);
+    return BGK_OK;
+}
+
+int bgk_col_MC(bgk_ctx *c) {
+    int r = check_ctx(c);
+    if (r) return r;
+    std::vector<Box> all{full_box(c->lat)};
+    r = collide_boxes(c, all, c->s0);
+    if (r) return r;
+    c->cur ^= 1;
+    c->steps++;
+    return BGK_OK;
+}
+
+int bgk_col_MC_masked(bgk_ctx *c, int opt) {
+    int r = check_ctx(c);
+    if (r) return r;
+    if (opt != 0 && opt != 1) return fail(BGK_EINVAL, "opt must be 0 (bulk) or 1 (border)");
+    std::vector<Box> boxes;
+    region_boxes(c, opt, boxes);
+    r = collide_boxes(c, boxes, c->s0);
+    if (r) return r;
+    if (opt == 1) {  // SRC/col_MC_masked.F90:369: swap after the border update
+        c->cur ^= 1;
+        c->steps++;
+    }
+    return BGK_OK;
+}
+
+int bgk_step(bgk_ctx *c, int nsteps) {
+    int r = check_ctx(c);
+    if (r) return r;
+    if (nsteps < 0) return fail(BGK_EINVAL, "nsteps must be >= 0");
+    hipEvent_t ev;
+    BGK_HIP(timing_begin(c, TCLS_STEP, c->s0, &ev));
+    r = (c->lat.layout == BGK_LAYOUT_SLAB_Z) ? step_slab(c, nsteps) : step_periodic(c, nsteps);
+    if (r) return r;
+    BGK_HIP(timing_end(c, TCLS_STEP, c->s0, ev));
+    return BGK_OK;
+}
+
+int bgk_diagno(bgk_ctx *c, int mode, double out[5]) {
+    int r = check_ctx(c);
+    if (r) return r;
+    if (!out) return fail(BGK_EINVAL, "null output");
+    if (mode != BGK_DIAG_CONSISTENT && mode != BGK_DIAG_REFCOMPAT)
+        return fail(BGK_EINVAL, "unknown diagnostic mode");
+    const Lattice &L = c->lat;
+    hipEvent_t ev;
+    BGK_HIP(timing_begin(c, TCLS_REDUCE, c->s0, &ev));
+    BGK_HIP(launch_diagno(L, c->cur, c->red_partials, c->red_blocks, c->red_out, c->s0));
+    BGK_HIP(timing_end(c, TCLS_REDUCE, c->s0, ev));
+    BGK_HIP(hipMemcpyAsync(c->red_host, c->red_out, 8 * sizeof(double), hipMemcpyDeviceToHost,
+                           c->s0));
+    BGK_HIP(hipStreamSynchronize(c->s0));
+    const double *v = c->red_host;
+    // SRC/diagno.F90:110 leaves ztot = ytot + zj(l,m,n) after the loop
+    const double ztot = (mode == BGK_DIAG_REFCOMPAT) ? v[2] + v[5] : v[3];
+    const double tot[5] = {v[0], v[1], v[2], ztot, v[4]};
+    for (int k = 0; k < 5; k++)  // SRC/diagno.F90:117-121 division order
+        out[k] = ((tot[k] / (double)L.nx) / (double)L.ny) / (double)L.nz;
+    for (int k = 0; k < 5; k++)
+        if (!std::isfinite(out[k]))
+            return fail(BGK_ENONFINITE, "diagno: non-finite moments (populations diverged)");
+    return BGK_OK;
+}
+
+int bgk_dissipation(bgk_ctx *c, double svisc, int mode, double out[2]) {
+    int r = check_ctx(c);
+    if (r) return r;
+    if (!out) return fail(BGK_EINVAL, "null output");
+    if (mode != BGK_DIAG_CONSISTENT && mode != BGK_DIAG_REFCOMPAT)
+        return fail(BGK_EINVAL, "unknown diagnostic mode");
+    const Lattice &L = c->lat;
+    const void *gc = nullptr;
+    if (L.layout == BGK_LAYOUT_SLAB_Z) {
+        // slab ghosts as the last exchange left them are the reference's
+        // view; consistent mode refreshes them first
+        if (mode == BGK_DIAG_CONSISTENT) {
+            r = halo_exchange(c, c->s0);
+            if (r) return r;
+        }
+    } else if (mode == BGK_DIAG_REFCOMPAT) {
+        if (!(c->flags & BGK_FLAG_REFCOMPAT_HALOS))
+            return fail(BGK_ESTATE,
+                        "refcompat dissipation needs a context created with "
+                        "BGK_FLAG_REFCOMPAT_HALOS");
+        gc = c->gcache[c->cur];
+    }
+    hipEvent_t ev;
+    BGK_HIP(timing_begin(c, TCLS_REDUCE, c->s0, &ev));
+    BGK_HIP(launch_dissipation(L, c->cur, gc, svisc, c->red_partials, c->red_blocks, c->red_out,
+                               c->s0));
+    BGK_HIP(timing_end(c, TCLS_REDUCE, c->s0, ev));
+    BGK_HIP(hipMemcpyAsync(c->red_host, c->red_out, 8 * sizeof(double), hipMemcpyDeviceToHost,
+                           c->s0));
+    BGK_HIP(hipStreamSynchronize(c->s0));
+    out[0] = c->red_host[0];
+    out[1] = c->red_host[1];
+    if (!std::isfinite(out[0]) || !std::isfinite(out[1]))
+        return fail(BGK_ENONFINITE, "dissipation: non-finite sums (populations diverged)");
+    return BGK_OK;
+}
+
+int bgk_synchronize(bgk_ctx *c) {
+    int r = check_ctx(c);
+    if (r) return r;
+    BGK_HIP(hipStreamSynchronize(c->s1));
+    BGK_HIP(hipStreamSynchronize(c->s0));
+    return BGK_OK;
+}
+
+int bgk_timing(bgk_ctx *c, double t[8], int reset) {
+    int r = check_ctx(c);
+    if (r) return r;
+    if (!t) return fail(BGK_EINVAL, "null output");
+    BGK_HIP(harvest(c));
+    Timing &T = c->timing;
+    t[0] = T.ms[TCLS_COLLIDE];
+    t[1] = (double)T.launches[TCLS_COLLIDE];
+    t[2] = T.ms[TCLS_REDUCE];
+    t[3] = (double)T.launches[TCLS_REDUCE];
+    t[4] = T.ms[TCLS_HALO];
+    t[5] = (double)T.launches[TCLS_HALO];
+    t[6] = T.ms[TCLS_STEP];
+    t[7] = (double)T.launches[TCLS_STEP];
+    if (reset) {
+        for (int k = 0; k < 4; k++) {
+            T.ms[k] = 0;
+            T.launches[k] = 0;
+        }
+    }
+    return BGK_OK;
+}
+
+int bgk_adjoint_ke(bgk_ctx *c, const double *u0, int nsteps, int ckpt_every, int levels,
+                   double *J, double *grad, double stats[4]) {
+    int r = check_ctx(c);
+    if (r) return r;
+    if (!u0 || !grad || !J) return fail(BGK_EINVAL, "null argument");
+    const size_t nb = sizeof(double) * 3 * c->lat.nsite;
+    double *du = nullptr, *dg = nullptr;
+    hipError_t e = hipMalloc(&du, nb);
+    if (e != hipSuccess) return hip_fail(e, "hipMalloc(u0)");
+    e = hipMalloc(&dg, nb);
+    if (e != hipSuccess) {
+        (void)hipFree(du);
+        return hip_fail(e, "hipMalloc(grad)");
+    }
+    e = hipMemcpy(du, u0, nb, hipMemcpyHostToDevice);
+    if (e != hipSuccess) r = hip_fail(e, "hipMemcpy(u0)");
+    if (!r) r = adjoint_ke(c, du, nsteps, ckpt_every, levels, J, dg, stats);
+    if (!r) {
+        e = hipMemcpy(grad, dg, nb, hipMemcpyDeviceToHost);
+        if (e != hipSuccess) r = hip_fail(e, "hipMemcpy(grad)");
+    }
+    (void)hipFree(du);
+    (void)hipFree(dg);
+    return r;
+}
+
+int bgk_adjoint_ke_device(bgk_ctx *c, const double *u0_dev, int nsteps, int ckpt_every,
+                          int levels, double *J, double *grad_dev, double stats[4]) {
+    int r = check_ctx(c);
+    if (r) return r;
+    if (!u0_dev || !grad_dev || !J) return fail(BGK_EINVAL, "null argument");
+    return adjoint_ke(c, u0_dev, nsteps, ckpt_every, levels, J, grad_dev, stats);
+}
+
+int bgk_forward_ke(bgk_ctx *c, const double *u0, int nsteps, double *J) {
+    int r = check_ctx(c);
+    if (r) return r;
+    if (!u0 || !J) return fail(BGK_EINVAL, "null argument");
+    const size_t nb = sizeof(double) * 3 * c->lat.nsite;
+    double *du = nullptr;
+    hipError_t e = hipMalloc(&du, nb);
+    if (e != hipSuccess) return hip_fail(e, "hipMalloc(u0)");
+    e = hipMemcpy(du, u0, nb, hipMemcpyHostToDevice);
+    if (e != hipSuccess) r = hip_fail(e, "hipMemcpy(u0)");
+    if (!r) r = forward_ke(c, du, nsteps, J);
+    (void)hipFree(du);
+    return r;
+}
+
+}  // extern "C"

@@ -1,0 +1,157 @@
+// bgk_internal.h -- device context and kernel launchers shared by the C-ABI
+// (bgk_capi.hip) and the kernels (bgk_kernels.hip, bgk_adjoint.hip,
+// bgk_hit.hip).  Not part of the public interface.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/bgk3d_hit.h"
+
+namespace bgk {
+
+// a box of interior sites [x0,x1) x [y0,y1) x [z0,z1), 0-based
+struct Box {
+    int x0, x1, y0, y1, z0, z1;
+    long long sites() const { return (long long)(x1 - x0) * (y1 - y0) * (z1 - z0); }
+};
+
+// Everything a kernel launch needs to find the populations.
+//   periodic layout: plane z of a population is at offset z*nx*ny
+//   slab layout    : interior plane z is at (z+1)*nx*ny, ghosts at 0, nz+1
+struct Lattice {
+    int nx, ny, nz;  // interior sites
+    int zoff;        // 0 periodic, 1 slab
+    int nzs;         // stored planes: nz or nz+2
+    int layout, storage;
+    int shifted;
+    long long nsite;       // nx*ny*nz
+    long long plane_elems; // nx*ny*nzs stored values per population
+    void *pop[2][19];      // population base pointers of copy 0/1; [c][18] is the
+                           // shared in-place rest population
+};
+
+struct Timing {
+    bool on = false;
+    std::vector<hipEvent_t> pool;  // reusable events
+    size_t used = 0;
+    struct Rec {
+        int cls;
+        hipEvent_t a, b;
+    };
+    std::vector<Rec> recs;
+    double ms[4] = {0, 0, 0, 0};
+    long long launches[4] = {0, 0, 0, 0};
+};
+
+}  // namespace bgk
+
+struct bgk_ctx {
+    bgk::Lattice lat;
+    int flags = 0;
+    double omega = 1.0;
+    int border = 10;
+    int cur = 0;  // which copy holds the current state
+    int device = 0;
+    long long steps = 0;
+    size_t bytes = 0;
+
+    hipStream_t s0 = nullptr;  // main stream (interior work)
+    hipStream_t s1 = nullptr;  // halo / boundary stream (slab overlap)
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+
+    void *pop_alloc = nullptr;  // one allocation holding 37 population planes
+
+    // slab halo messages: [face 0 = z+ (plane nz), 1 = z- (plane 1)][5][nx*ny]
+    void *halo_send = nullptr, *halo_recv = nullptr;
+    size_t halo_bytes = 0;
+
+    // stale-halo images (reference ghost semantics), per copy:
+    // gx[19][ny*nz], gy[19][nx*nz], gz[19][nx*ny]
+    void *gcache[2] = {nullptr, nullptr};
+    size_t gcache_bytes = 0;
+
+    // reductions
+    double *red_partials = nullptr;  // [max_blocks][8]
+    double *red_out = nullptr;       // [8] device
+    double *red_host = nullptr;      // [8] pinned host
+    int red_blocks = 0;
+
+    // staging for host <-> device population transfers (one plane of doubles)
+    double *staging = nullptr;
+
+    bgk::Timing timing;
+};
+
+namespace bgk {
+
+void set_error(const std::string &msg);
+int fail(int code, const std::string &msg);
+int hip_fail(hipError_t e, const char *what);
+
+#define BGK_HIP(call)                                 \
+    do {                                              \
+        hipError_t e_ = (call);                       \
+        if (e_ != hipSuccess) return hip_fail(e_, #call); \
+    } while (0)
+
+size_t storage_bytes(int storage);
+
+// timing helpers (no-ops unless BGK_FLAG_TIMING)
+enum { TCLS_COLLIDE = 0, TCLS_REDUCE = 1, TCLS_HALO = 2, TCLS_STEP = 3 };
+hipError_t timing_begin(bgk_ctx *c, int cls, hipStream_t s, hipEvent_t *out);
+hipError_t timing_end(bgk_ctx *c, int cls, hipStream_t s, hipEvent_t begin);
+
+// ---- launchers (bgk_kernels.hip) ----
+// fused pull-stream + collide from copy `src` to copy 1-src over `box`
+hipError_t launch_collide(const Lattice &L, int src, const Box &box, double omega,
+                          hipStream_t s);
+// reductions: at-site (diagno) and pulled (dissipation)
+//   mode 0: pulled gather with periodic wrap (or slab ghosts as stored)
+//   mode 1: pulled gather reading face images from gcache (periodic layout)
+hipError_t launch_diagno(const Lattice &L, int cur, double *partials, int nblocks, double *out,
+                         hipStream_t s);
+hipError_t launch_dissipation(const Lattice &L, int cur, const void *gcache, double svisc,
+                              double *partials, int nblocks, double *out, hipStream_t s);
+// slab halo path
+hipError_t launch_pack_z(const Lattice &L, int cur, void *send, hipStream_t s);
+hipError_t launch_unpack_z(const Lattice &L, int cur, const void *recv, hipStream_t s);
+// stale-halo images of the periodic layout
+hipError_t launch_pack_faces(const Lattice &L, int cur, void *gcache, hipStream_t s);
+// population transfer: interior sites i0 .. i0+count-1 of one population
+// from/to a device array of doubles
+hipError_t launch_plane_from_f64(const Lattice &L, int copy, int q, const double *src,
+                                 long long i0, long long count, hipStream_t s);
+hipError_t launch_plane_to_f64(const Lattice &L, int copy, int q, double *dst, long long i0,
+                               long long count, hipStream_t s);
+// equilibrium of a device velocity field u = 3 planes of `ld` doubles
+// describing sites i0 .. i0+count-1
+hipError_t launch_init_equilibrium(const Lattice &L, int copy, const double *u, long long ld,
+                                   long long i0, long long count, hipStream_t s);
+// zero the ghost planes of the slab layout of one copy
+hipError_t launch_fill_zero(void *p, size_t bytes, hipStream_t s);
+
+// ---- HIT synthesis (bgk_hit.cpp / bgk_hit.hip) ----
+struct HitSpectrum {
+    int Kx, Ky, Kz;                 // mode box half widths
+    std::vector<double> coef;       // [3][2Kz+1][2Ky+1][Kx+1] complex (re,im) pairs
+    std::vector<double> twx, twy, twz;  // cos,sin tables: 2*lx, 2*ly, 2*lz
+};
+int hit_spectrum(int lx, int ly, int lz, double k0, double urms, uint64_t seed, HitSpectrum &sp);
+void hit_synthesize_host(int lx, int ly, int lz, const HitSpectrum &sp, double *u);
+int hit_synthesize_device(bgk_ctx *c, const HitSpectrum &sp, double *u_dev);
+
+// ---- adjoint (bgk_adjoint.hip) ----
+int adjoint_ke(bgk_ctx *c, const double *u0_dev, int nsteps, int ckpt_every, int levels,
+               double *J, double *grad_dev, double stats[4]);
+int forward_ke(bgk_ctx *c, const double *u0_dev, int nsteps, double *J);
+
+// shared helpers used by the adjoint
+int reduce_ke(bgk_ctx *c, int copy, double *J);
+int step_periodic(bgk_ctx *c, int nsteps);
+
+}  // namespace bgk

@@ -1,0 +1,505 @@
+// bgk_kernels.hip -- gfx950 kernels of the BGK3D_HIT hot path.
+//
+//   k_collide          fused pull stream + BGK collide (SRC/col_MC.F90:104-361),
+//                      one thread per site, x-contiguous 64x4 tiles, periodic
+//                      wrap in the index math (or slab ghost planes in z)
+//   k_diagno           at-site moment sums (SRC/diagno.F90:73-115)
+//   k_dissipation      pulled Pi^neq / kinetic energy sums
+//                      (SRC/dissipation.F90:70-253)
+//   k_pack_z/unpack_z  slab halo messages of the 5 crossing populations per
+//                      z face (SRC/bcond.comm.step3.F90:79-143)
+//   k_pack_faces       stale-halo images for the reference-compatible
+//                      dissipation (periodic layout)
+//   k_init_equilibrium population initialisation (SRC/init.F90:97-137)
+//
+// All reductions accumulate in FP64 with a fixed grid and a fixed
+// combination order, so they are deterministic run to run.
+#include "bgk_internal.h"
+#include "d3q19.h"
+
+namespace bgk {
+
+template <typename S>
+struct Pops {
+    S *p[19];
+};
+
+template <typename S>
+static Pops<S> pops_of(const Lattice &L, int copy) {
+    Pops<S> P;
+    for (int q = 0; q < 19; q++) P.p[q] = (S *)L.pop[copy][q];
+    return P;
+}
+
+// Launch geometry of a box of sites in 64 x 4 tiles (one z plane deep).
+struct Geo {
+    int nx, ny, nz, nxy, zoff, periodic_z;
+    int tiles_x, tiles_y;
+    int x0, x1, y0, y1, z0;
+};
+
+static Geo make_geo(const Lattice &L, const Box &b, int *ntiles) {
+    Geo g;
+    g.nx = L.nx;
+    g.ny = L.ny;
+    g.nz = L.nz;
+    g.nxy = L.nx * L.ny;
+    g.zoff = L.zoff;
+    g.periodic_z = (L.layout == BGK_LAYOUT_PERIODIC);
+    g.tiles_x = (b.x1 - b.x0 + 63) / 64;
+    g.tiles_y = (b.y1 - b.y0 + 3) / 4;
+    g.x0 = b.x0;
+    g.x1 = b.x1;
+    g.y0 = b.y0;
+    g.y1 = b.y1;
+    g.z0 = b.z0;
+    *ntiles = g.tiles_x * g.tiles_y * (b.z1 - b.z0);
+    return g;
+}
+
+// Workgroups are dispatched round-robin over the 8 XCDs.  Remap so that each
+// XCD works on one contiguous range of tiles: x-adjacent tiles then share an
+// L2 and the cache line that straddles their boundary is fetched once.
+__device__ __forceinline__ int xcd_remap(int b, int G) {
+    const int q = G >> 3, r = G & 7;
+    const int xcd = b & 7, slot = b >> 3;
+    return (xcd < r) ? xcd * (q + 1) + slot : r * (q + 1) + (xcd - r) * q + slot;
+}
+
+// ------------------------------------------------------------------ collide
+template <typename S, typename C>
+__global__ __launch_bounds__(256) void k_collide(Pops<S> src, Pops<S> dst, Geo g, C omega,
+                                                 C cte1) {
+    const int t = xcd_remap((int)blockIdx.x, (int)gridDim.x);
+    const int tx = t % g.tiles_x;
+    const int r = t / g.tiles_x;
+    const int ty = r % g.tiles_y;
+    const int tz = r / g.tiles_y;
+    const int x = g.x0 + tx * 64 + (int)threadIdx.x;
+    const int y = g.y0 + ty * 4 + (int)threadIdx.y;
+    const int z = g.z0 + tz;
+    if (x >= g.x1 || y >= g.y1) return;
+
+    const int xm = (x == 0) ? g.nx - 1 : x - 1;
+    const int xp = (x == g.nx - 1) ? 0 : x + 1;
+    const int ym = (y == 0) ? g.ny - 1 : y - 1;
+    const int yp = (y == g.ny - 1) ? 0 : y + 1;
+    const int zs = z + g.zoff;
+    int zm, zp;
+    if (g.periodic_z) {
+        zm = (z == 0) ? g.nz - 1 : z - 1;
+        zp = (z == g.nz - 1) ? 0 : z + 1;
+    } else {  // slab: ghost planes 0 and nz+1 hold the neighbours' images
+        zm = zs - 1;
+        zp = zs + 1;
+    }
+    const int X[3] = {xm, x, xp};
+    const int Y[3] = {ym * g.nx, y * g.nx, yp * g.nx};
+    const int Z[3] = {zm * g.nxy, zs * g.nxy, zp * g.nxy};
+
+    C f[19];
+#pragma unroll
+    for (int q = 0; q < 19; q++) {
+        // pull: population q arrives from x - c_q
+        const int idx = X[1 - cx(q)] + Y[1 - cy(q)] + Z[1 - cz(q)];
+        f[q] = Storage<S>::template load<C>(src.p[q][idx]);
+    }
+    C b[19];
+    collide_site<C>(f, b, omega, cte1);
+    const int c = x + Y[1] + Z[1];
+#pragma unroll
+    for (int q = 0; q < 18; q++) dst.p[q][c] = Storage<S>::template store<C>(b[q]);
+    src.p[18][c] = Storage<S>::template store<C>(b[18]);  // rest: in place
+}
+
+template <typename S, typename C>
+static hipError_t collide_t(const Lattice &L, int src, const Box &box, double omega,
+                            hipStream_t s) {
+    if (box.sites() <= 0) return hipSuccess;
+    int ntiles;
+    Geo g = make_geo(L, box, &ntiles);
+    Pops<S> a = pops_of<S>(L, src), b = pops_of<S>(L, 1 - src);
+    hipLaunchKernelGGL((k_collide<S, C>), dim3(ntiles), dim3(64, 4), 0, s, a, b, g, (C)omega,
+                       (C)(L.shifted ? 1.0 : 0.0));
+    return hipGetLastError();
+}
+
+// --------------------------------------------------------------- reductions
+constexpr int RED_THREADS = 256;
+constexpr int RED_NV = 8;
+
+// Reduce NV doubles over the block and write them to out[0..NV).
+template <int NV>
+__device__ __forceinline__ void block_sum_store(double (&v)[NV], double *out) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+#pragma unroll
+        for (int k = 0; k < NV; k++) v[k] += __shfl_xor(v[k], off);
+    }
+    __shared__ double sm[RED_THREADS / 64][NV];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) {
+#pragma unroll
+        for (int k = 0; k < NV; k++) sm[wave][k] = v[k];
+    }
+    __syncthreads();
+    if ((int)threadIdx.x < NV) {
+        double s = 0.0;
+#pragma unroll
+        for (int w = 0; w < RED_THREADS / 64; w++) s += sm[w][threadIdx.x];
+        out[threadIdx.x] = s;
+    }
+}
+
+__global__ __launch_bounds__(RED_THREADS) void k_finalize(const double *partials, int nblocks,
+                                                          double *out) {
+    double v[RED_NV];
+#pragma unroll
+    for (int k = 0; k < RED_NV; k++) v[k] = 0.0;
+    for (int b = threadIdx.x; b < nblocks; b += RED_THREADS) {
+#pragma unroll
+        for (int k = 0; k < RED_NV; k++) v[k] += partials[(size_t)b * RED_NV + k];
+    }
+    block_sum_store<RED_NV>(v, out);
+}
+
+// at-site moments: v = {sum rho, sum ux, sum uy, sum uz, sum |u|^2/2,
+// uz of the last site (l,m,n)}
+template <typename S, typename C>
+__global__ __launch_bounds__(RED_THREADS) void k_diagno(Pops<S> P, int nx, int ny, int nz,
+                                                        int zoff, C cte1, double *partials) {
+    double v[RED_NV];
+#pragma unroll
+    for (int k = 0; k < RED_NV; k++) v[k] = 0.0;
+    const long long nsite = (long long)nx * ny * nz;
+    const long long off = (long long)zoff * nx * ny;
+    for (long long i = (long long)blockIdx.x * RED_THREADS + threadIdx.x; i < nsite;
+         i += (long long)gridDim.x * RED_THREADS) {
+        C f[19];
+#pragma unroll
+        for (int q = 0; q < 19; q++) f[q] = Storage<S>::template load<C>(P.p[q][off + i]);
+        C rho, ux, uy, uz;
+        moments_site<C>(f, cte1, rho, ux, uy, uz);
+        const C sk = (C)0.5 * (ux * ux + uy * uy + uz * uz);
+        v[0] += (double)rho;
+        v[1] += (double)ux;
+        v[2] += (double)uy;
+        v[3] += (double)uz;
+        v[4] += (double)sk;
+        if (i == nsite - 1) v[5] = (double)uz;
+    }
+    block_sum_store<RED_NV>(v, partials + (size_t)blockIdx.x * RED_NV);
+}
+
+// pulled moments + Pi^neq; GC = 0: sources wrap periodically (slab: read
+// the ghost planes as stored), GC = 1: sources outside the box read the
+// recorded face images gx/gy/gz (reference stale-halo semantics)
+template <typename S, typename C, int GC>
+__global__ __launch_bounds__(RED_THREADS) void k_dissipation(Pops<S> P, const S *gcache, int nx,
+                                                             int ny, int nz, int zoff,
+                                                             int periodic_z, C cte1, C svisc,
+                                                             double *partials) {
+    double v[RED_NV];
+#pragma unroll
+    for (int k = 0; k < RED_NV; k++) v[k] = 0.0;
+    const int nxy = nx * ny;
+    const long long nsite = (long long)nxy * nz;
+    const S *gx = gcache;
+    const S *gy = gcache + (size_t)19 * ny * nz;
+    const S *gz = gy + (size_t)19 * nx * nz;
+    for (long long i = (long long)blockIdx.x * RED_THREADS + threadIdx.x; i < nsite;
+         i += (long long)gridDim.x * RED_THREADS) {
+        const int x = (int)(i % nx);
+        const int y = (int)((i / nx) % ny);
+        const int z = (int)(i / nxy);
+        C f[19];
+#pragma unroll
+        for (int q = 0; q < 19; q++) {
+            int sx = x - cx(q), sy = y - cy(q), sz = z - cz(q);
+            const bool ox = (sx < 0 || sx >= nx), oy = (sy < 0 || sy >= ny);
+            const bool oz = (sz < 0 || sz >= nz);
+            const int wx = sx < 0 ? sx + nx : (sx >= nx ? sx - nx : sx);
+            const int wy = sy < 0 ? sy + ny : (sy >= ny ? sy - ny : sy);
+            const int wz = sz < 0 ? sz + nz : (sz >= nz ? sz - nz : sz);
+            S raw;
+            if (GC && ox) {
+                raw = gx[(size_t)q * ny * nz + wy + (size_t)ny * wz];
+            } else if (GC && oy) {
+                raw = gy[(size_t)q * nx * nz + wx + (size_t)nx * wz];
+            } else if (GC && oz) {
+                raw = gz[(size_t)q * nxy + wx + (size_t)nx * wy];
+            } else {
+                const int zz = periodic_z ? wz : sz + zoff;
+                raw = P.p[q][wx + nx * wy + (size_t)nxy * zz];
+            }
+            f[q] = Storage<S>::template load<C>(raw);
+        }
+        C dsite, tsite;
+        dissipation_site<C>(f, cte1, svisc, dsite, tsite);
+        v[0] += (double)dsite;
+        v[1] += (double)tsite;
+    }
+    block_sum_store<RED_NV>(v, partials + (size_t)blockIdx.x * RED_NV);
+}
+
+template <typename S, typename C>
+static hipError_t diagno_t(const Lattice &L, int cur, double *partials, int nblocks, double *out,
+                           hipStream_t s) {
+    hipLaunchKernelGGL((k_diagno<S, C>), dim3(nblocks), dim3(RED_THREADS), 0, s,
+                       pops_of<S>(L, cur), L.nx, L.ny, L.nz, L.zoff, (C)(L.shifted ? 1.0 : 0.0),
+                       partials);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(RED_THREADS), 0, s, partials, nblocks, out);
+    return hipGetLastError();
+}
+
+template <typename S, typename C>
+static hipError_t dissipation_t(const Lattice &L, int cur, const void *gcache, double svisc,
+                                double *partials, int nblocks, double *out, hipStream_t s) {
+    const C cte1 = (C)(L.shifted ? 1.0 : 0.0);
+    const int pz = (L.layout == BGK_LAYOUT_PERIODIC);
+    if (gcache)
+        hipLaunchKernelGGL((k_dissipation<S, C, 1>), dim3(nblocks), dim3(RED_THREADS), 0, s,
+                           pops_of<S>(L, cur), (const S *)gcache, L.nx, L.ny, L.nz, L.zoff, pz,
+                           cte1, (C)svisc, partials);
+    else
+        hipLaunchKernelGGL((k_dissipation<S, C, 0>), dim3(nblocks), dim3(RED_THREADS), 0, s,
+                           pops_of<S>(L, cur), (const S *)nullptr, L.nx, L.ny, L.nz, L.zoff, pz,
+                           cte1, (C)svisc, partials);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(RED_THREADS), 0, s, partials, nblocks, out);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------- halo path
+// crossing populations (0-based) per z face, SRC/bcond.comm.step3.F90:82-140
+__constant__ int c_zplus[5] = {3, 5, 6, 12, 17};   // 04 06 07 13 18: leave through +z
+__constant__ int c_zminus[5] = {1, 8, 10, 14, 15}; // 02 09 11 15 16: leave through -z
+
+// message layout: [face][5][nx*ny], face 0 = the +z message (top interior
+// plane nz), face 1 = the -z message (bottom interior plane 1)
+template <typename S>
+__global__ __launch_bounds__(256) void k_pack_z(Pops<S> P, int nxy, int nz, S *send) {
+    const int p = blockIdx.x * 256 + threadIdx.x;
+    if (p >= nxy) return;
+    const int q5 = blockIdx.y;  // 0..4
+    send[(size_t)(0 * 5 + q5) * nxy + p] = P.p[c_zplus[q5]][(size_t)nz * nxy + p];
+    send[(size_t)(1 * 5 + q5) * nxy + p] = P.p[c_zminus[q5]][(size_t)1 * nxy + p];
+}
+
+// the +z message fills the bottom ghost plane 0, the -z message the top
+// ghost plane nz+1 (SRC/bcond.comm.step3.F90:97-106, 133-142)
+template <typename S>
+__global__ __launch_bounds__(256) void k_unpack_z(Pops<S> P, int nxy, int nz, const S *recv) {
+    const int p = blockIdx.x * 256 + threadIdx.x;
+    if (p >= nxy) return;
+    const int q5 = blockIdx.y;
+    P.p[c_zplus[q5]][p] = recv[(size_t)(0 * 5 + q5) * nxy + p];
+    P.p[c_zminus[q5]][(size_t)(nz + 1) * nxy + p] = recv[(size_t)(1 * 5 + q5) * nxy + p];
+}
+
+template <typename S>
+static hipError_t pack_z_t(const Lattice &L, int cur, void *send, hipStream_t s) {
+    const int nxy = L.nx * L.ny;
+    hipLaunchKernelGGL((k_pack_z<S>), dim3((nxy + 255) / 256, 5), dim3(256), 0, s,
+                       pops_of<S>(L, cur), nxy, L.nz, (S *)send);
+    return hipGetLastError();
+}
+template <typename S>
+static hipError_t unpack_z_t(const Lattice &L, int cur, const void *recv, hipStream_t s) {
+    const int nxy = L.nx * L.ny;
+    hipLaunchKernelGGL((k_unpack_z<S>), dim3((nxy + 255) / 256, 5), dim3(256), 0, s,
+                       pops_of<S>(L, cur), nxy, L.nz, (const S *)recv);
+    return hipGetLastError();
+}
+
+// face images of the periodic layout: gx[q] = plane x = (c_x>0 ? nx-1 : 0)
+// of population q for the 10 populations with c_x != 0, likewise gy, gz.
+template <typename S>
+__global__ __launch_bounds__(256) void k_pack_faces(Pops<S> P, int nx, int ny, int nz, S *g) {
+    const int axis = blockIdx.y;
+    const long long p = (long long)blockIdx.x * 256 + threadIdx.x;
+    const int nxy = nx * ny;
+    S *gx = g;
+    S *gy = g + (size_t)19 * ny * nz;
+    S *gz = gy + (size_t)19 * nx * nz;
+    if (axis == 0) {
+        if (p >= (long long)ny * nz) return;
+        const int y = (int)(p % ny), z = (int)(p / ny);
+#pragma unroll
+        for (int q = 0; q < 18; q++) {
+            if (cx(q) == 0) continue;
+            const int x = cx(q) > 0 ? nx - 1 : 0;
+            gx[(size_t)q * ny * nz + p] = P.p[q][x + nx * y + (size_t)nxy * z];
+        }
+    } else if (axis == 1) {
+        if (p >= (long long)nx * nz) return;
+        const int x = (int)(p % nx), z = (int)(p / nx);
+#pragma unroll
+        for (int q = 0; q < 18; q++) {
+            if (cy(q) == 0) continue;
+            const int y = cy(q) > 0 ? ny - 1 : 0;
+            gy[(size_t)q * nx * nz + p] = P.p[q][x + nx * y + (size_t)nxy * z];
+        }
+    } else {
+        if (p >= (long long)nxy) return;
+#pragma unroll
+        for (int q = 0; q < 18; q++) {
+            if (cz(q) == 0) continue;
+            const int z = cz(q) > 0 ? nz - 1 : 0;
+            gz[(size_t)q * nxy + p] = P.p[q][p + (size_t)nxy * z];
+        }
+    }
+}
+
+template <typename S>
+static hipError_t pack_faces_t(const Lattice &L, int cur, void *g, hipStream_t s) {
+    long long mx = (long long)L.ny * L.nz;
+    if ((long long)L.nx * L.nz > mx) mx = (long long)L.nx * L.nz;
+    if ((long long)L.nx * L.ny > mx) mx = (long long)L.nx * L.ny;
+    hipLaunchKernelGGL((k_pack_faces<S>), dim3((unsigned)((mx + 255) / 256), 3), dim3(256), 0, s,
+                       pops_of<S>(L, cur), L.nx, L.ny, L.nz, (S *)g);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------- transfers and init
+// Host transfers go through a staging buffer in chunks of `count` sites
+// starting at interior site i0.
+template <typename S, typename C>
+__global__ __launch_bounds__(256) void k_plane_from_f64(S *dst, const double *src,
+                                                        long long count) {
+    const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (i < count) dst[i] = Storage<S>::template store<C>((C)src[i]);
+}
+template <typename S, typename C>
+__global__ __launch_bounds__(256) void k_plane_to_f64(const S *src, double *dst, long long count) {
+    const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (i < count) dst[i] = (double)Storage<S>::template load<C>(src[i]);
+}
+
+template <typename S, typename C>
+static hipError_t from_f64_t(const Lattice &L, int copy, int q, const double *src, long long i0,
+                             long long count, hipStream_t s) {
+    S *dst = (S *)L.pop[copy][q] + (size_t)L.zoff * L.nx * L.ny + i0;
+    hipLaunchKernelGGL((k_plane_from_f64<S, C>), dim3((unsigned)((count + 255) / 256)),
+                       dim3(256), 0, s, dst, src, count);
+    return hipGetLastError();
+}
+template <typename S, typename C>
+static hipError_t to_f64_t(const Lattice &L, int copy, int q, double *dst, long long i0,
+                           long long count, hipStream_t s) {
+    const S *src = (const S *)L.pop[copy][q] + (size_t)L.zoff * L.nx * L.ny + i0;
+    hipLaunchKernelGGL((k_plane_to_f64<S, C>), dim3((unsigned)((count + 255) / 256)),
+                       dim3(256), 0, s, src, dst, count);
+    return hipGetLastError();
+}
+
+// SRC/init.F90:97-137 with crho = 1: populations of the equilibrium of u.
+// u holds 3 planes (ux, uy, uz) of `ld` values for sites i0 .. i0+count-1.
+template <typename S, typename C>
+__global__ __launch_bounds__(256) void k_init_equilibrium(Pops<S> P, const double *u,
+                                                          long long ld, long long count,
+                                                          long long dst0, C cte1) {
+    const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (i >= count) return;
+    const C rf = 3, qf = 1.5, tre = 3, crho = 1, zero = 0;
+    const C p0 = K<C>::p0, p1 = K<C>::p1, p2 = K<C>::p2;
+    const C xj = (C)u[i], yj = (C)u[ld + i], zj = (C)u[2 * ld + i];
+    const C cvsq = xj * xj + yj * yj + zj * zj;
+    C c[19];
+    c[0] = rf * (xj - yj) + qf * (tre * (xj - yj) * (xj - yj) - cvsq);
+    c[1] = rf * (xj - zj) + qf * (tre * (xj - zj) * (xj - zj) - cvsq);
+    c[2] = rf * (xj + yj) + qf * (tre * (xj + yj) * (xj + yj) - cvsq);
+    c[3] = rf * (xj + zj) + qf * (tre * (xj + zj) * (xj + zj) - cvsq);
+    c[4] = rf * (xj) + qf * (tre * (xj) * (xj)-cvsq);
+    c[5] = rf * (zj) + qf * (tre * (zj) * (zj)-cvsq);
+    c[6] = rf * (yj + zj) + qf * (tre * (yj + zj) * (yj + zj) - cvsq);
+    c[7] = rf * (yj) + qf * (tre * (yj) * (yj)-cvsq);
+    c[8] = rf * (yj - zj) + qf * (tre * (yj - zj) * (yj - zj) - cvsq);
+    c[9] = rf * (-xj - yj) + qf * (tre * (xj + yj) * (xj + yj) - cvsq);
+    c[10] = rf * (-xj - zj) + qf * (tre * (xj + zj) * (xj + zj) - cvsq);
+    c[11] = rf * (-xj + yj) + qf * (tre * (xj - yj) * (xj - yj) - cvsq);
+    c[12] = rf * (-xj + zj) + qf * (tre * (xj - zj) * (xj - zj) - cvsq);
+    c[13] = rf * (-xj) + qf * (tre * (xj) * (xj)-cvsq);
+    c[14] = rf * (-zj) + qf * (tre * (zj) * (zj)-cvsq);
+    c[15] = rf * (-yj - zj) + qf * (tre * (yj + zj) * (yj + zj) - cvsq);
+    c[16] = rf * (-yj) + qf * (tre * (yj) * (yj)-cvsq);
+    c[17] = rf * (-yj + zj) + qf * (tre * (yj - zj) * (yj - zj) - cvsq);
+    c[18] = rf * (zero) + qf * (tre * (zero) * (zero)-cvsq);
+#pragma unroll
+    for (int q = 0; q < 19; q++) {
+        const C w = (q == 18) ? p0 : ((cx(q) * cx(q) + cy(q) * cy(q) + cz(q) * cz(q)) == 1 ? p1 : p2);
+        P.p[q][dst0 + i] = Storage<S>::template store<C>(crho * w * (cte1 + c[q]));
+    }
+}
+
+template <typename S, typename C>
+static hipError_t init_eq_t(const Lattice &L, int copy, const double *u, long long ld,
+                            long long i0, long long count, hipStream_t s) {
+    // init.F90 uses cte1 = 0 when shifted (SRC/init.F90:65-69)
+    hipLaunchKernelGGL((k_init_equilibrium<S, C>), dim3((unsigned)((count + 255) / 256)),
+                       dim3(256), 0, s, pops_of<S>(L, copy), u, ld, count,
+                       (long long)L.zoff * L.nx * L.ny + i0, (C)(L.shifted ? 0.0 : 1.0));
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------ kind dispatch
+#define BGK_DISPATCH(storage, FN, ...)                                   \
+    switch (storage) {                                                   \
+    case BGK_STORE_F64: return FN<double, double>(__VA_ARGS__);          \
+    case BGK_STORE_F32: return FN<float, float>(__VA_ARGS__);            \
+    case BGK_STORE_F32_MIXED: return FN<float, double>(__VA_ARGS__);     \
+    case BGK_STORE_F16: return FN<half_t, float>(__VA_ARGS__);           \
+    case BGK_STORE_BF16: return FN<bf16_t, float>(__VA_ARGS__);          \
+    default: return hipErrorInvalidValue;                                \
+    }
+#define BGK_DISPATCH_S(storage, FN, ...)                                 \
+    switch (storage) {                                                   \
+    case BGK_STORE_F64: return FN<double>(__VA_ARGS__);                  \
+    case BGK_STORE_F32:                                                  \
+    case BGK_STORE_F32_MIXED: return FN<float>(__VA_ARGS__);             \
+    case BGK_STORE_F16: return FN<half_t>(__VA_ARGS__);                  \
+    case BGK_STORE_BF16: return FN<bf16_t>(__VA_ARGS__);                 \
+    default: return hipErrorInvalidValue;                                \
+    }
+
+hipError_t launch_collide(const Lattice &L, int src, const Box &box, double omega,
+                          hipStream_t s) {
+    BGK_DISPATCH(L.storage, collide_t, L, src, box, omega, s);
+}
+hipError_t launch_diagno(const Lattice &L, int cur, double *partials, int nblocks, double *out,
+                         hipStream_t s) {
+    BGK_DISPATCH(L.storage, diagno_t, L, cur, partials, nblocks, out, s);
+}
+hipError_t launch_dissipation(const Lattice &L, int cur, const void *gcache, double svisc,
+                              double *partials, int nblocks, double *out, hipStream_t s) {
+    BGK_DISPATCH(L.storage, dissipation_t, L, cur, gcache, svisc, partials, nblocks, out, s);
+}
+hipError_t launch_pack_z(const Lattice &L, int cur, void *send, hipStream_t s) {
+    BGK_DISPATCH_S(L.storage, pack_z_t, L, cur, send, s);
+}
+hipError_t launch_unpack_z(const Lattice &L, int cur, const void *recv, hipStream_t s) {
+    BGK_DISPATCH_S(L.storage, unpack_z_t, L, cur, recv, s);
+}
+hipError_t launch_pack_faces(const Lattice &L, int cur, void *gcache, hipStream_t s) {
+    BGK_DISPATCH_S(L.storage, pack_faces_t, L, cur, gcache, s);
+}
+hipError_t launch_plane_from_f64(const Lattice &L, int copy, int q, const double *src,
+                                 long long i0, long long count, hipStream_t s) {
+    BGK_DISPATCH(L.storage, from_f64_t, L, copy, q, src, i0, count, s);
+}
+hipError_t launch_plane_to_f64(const Lattice &L, int copy, int q, double *dst, long long i0,
+                               long long count, hipStream_t s) {
+    BGK_DISPATCH(L.storage, to_f64_t, L, copy, q, dst, i0, count, s);
+}
+hipError_t launch_init_equilibrium(const Lattice &L, int copy, const double *u, long long ld,
+                                   long long i0, long long count, hipStream_t s) {
+    BGK_DISPATCH(L.storage, init_eq_t, L, copy, u, ld, i0, count, s);
+}
+hipError_t launch_fill_zero(void *p, size_t bytes, hipStream_t s) {
+    return hipMemsetAsync(p, 0, bytes, s);
+}
+
+}  // namespace bgk

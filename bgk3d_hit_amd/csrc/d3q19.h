@@ -1,0 +1,290 @@
+// d3q19.h -- D3Q19 lattice constants, storage traits and the per-site BGK
+// collision shared by every kernel of the library.
+//
+// Velocity set and weights: reference SRC/storage.F90:98-133 (p0 = 1/3,
+// p1 = 1/18, p2 = 1/36) and SRC/alloca.F90:57-175; population q (1..19) is
+// stored at index q-1.  Pull streaming: population q at site x comes from
+// site x - c_q (SRC/col_MC.F90:113-131).
+//
+// The arithmetic of collide_site follows SRC/col_MC.F90:133-219,333-351
+// term by term in textual left-to-right order; the library is compiled with
+// -ffp-contract=off so that FP64/FP32 results are reproducible bit for bit
+// on any IEEE machine that evaluates the same expressions.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace bgk {
+
+// c_q for q = 1..19 at index q-1
+__host__ __device__ constexpr int cx(int i) {
+    constexpr int v[19] = {1, 1, 1, 1, 1, 0, 0, 0, 0, -1, -1, -1, -1, -1, 0, 0, 0, 0, 0};
+    return v[i];
+}
+__host__ __device__ constexpr int cy(int i) {
+    constexpr int v[19] = {-1, 0, 1, 0, 0, 0, 1, 1, 1, -1, 0, 1, 0, 0, 0, -1, -1, -1, 0};
+    return v[i];
+}
+__host__ __device__ constexpr int cz(int i) {
+    constexpr int v[19] = {0, -1, 0, 1, 0, 1, 1, 0, -1, 0, -1, 0, 1, 0, -1, -1, 0, 1, 0};
+    return v[i];
+}
+// weight of population i (0-based): 1/3 rest, 1/18 axis, 1/36 diagonal
+__host__ __device__ constexpr double wq(int i) {
+    return (cx(i) * cx(i) + cy(i) * cy(i) + cz(i) * cz(i)) == 0
+               ? 1.0 / 3.0
+               : ((cx(i) * cx(i) + cy(i) * cy(i) + cz(i) * cz(i)) == 1 ? 1.0 / 18.0 : 1.0 / 36.0);
+}
+
+// ---------------------------------------------------------------- storage
+// Storage element types.  Half and bfloat16 are kept as raw 16-bit words
+// in memory and converted with round-to-nearest-even.
+struct half_t {
+    uint16_t bits;
+};
+struct bf16_t {
+    uint16_t bits;
+};
+
+template <typename S>
+struct Storage;
+
+template <>
+struct Storage<double> {
+    using raw = double;
+    template <typename C>
+    __device__ __forceinline__ static C load(double v) { return (C)v; }
+    template <typename C>
+    __device__ __forceinline__ static double store(C v) { return (double)v; }
+};
+template <>
+struct Storage<float> {
+    using raw = float;
+    template <typename C>
+    __device__ __forceinline__ static C load(float v) { return (C)v; }
+    template <typename C>
+    __device__ __forceinline__ static float store(C v) { return (float)v; }
+};
+template <>
+struct Storage<half_t> {
+    using raw = half_t;
+    template <typename C>
+    __device__ __forceinline__ static C load(half_t v) {
+        return (C)(float)__builtin_bit_cast(_Float16, v.bits);
+    }
+    template <typename C>
+    __device__ __forceinline__ static half_t store(C v) {
+        // v_cvt_f16_f32: IEEE round to nearest even (default mode)
+        return half_t{__builtin_bit_cast(uint16_t, (_Float16)(float)v)};
+    }
+};
+template <>
+struct Storage<bf16_t> {
+    using raw = bf16_t;
+    template <typename C>
+    __device__ __forceinline__ static C load(bf16_t v) {
+        return (C)__builtin_bit_cast(float, (uint32_t)v.bits << 16);
+    }
+    template <typename C>
+    __device__ __forceinline__ static bf16_t store(C v) {
+        uint32_t x = __builtin_bit_cast(uint32_t, (float)v);
+        if ((x & 0x7fffffffu) > 0x7f800000u) return bf16_t{(uint16_t)((x >> 16) | 0x40u)};
+        x += 0x7fffu + ((x >> 16) & 1u);
+        return bf16_t{(uint16_t)(x >> 16)};
+    }
+};
+
+// ---------------------------------------------------------------- collide
+// Lattice constants in the compute kind: the reference casts double
+// parameters to mykind (SRC/storage.F90:123-133).
+template <typename C>
+struct K {
+    static constexpr C uno = (C)1.0, tre = (C)3.0, rf = (C)3.0, qf = (C)1.5;
+    static constexpr C p0 = (C)(1.0 / 3.0);
+    static constexpr C p1 = (C)(1.0 / 18.0);
+    static constexpr C p2 = (C)(1.0 / (3.0 * 3.0 * (1.0 + 3.0)));
+};
+
+// One site of SRC/col_MC.F90:133-219,333-351.  x[] are the pulled
+// populations, b[] receives the post-collision populations (b[18] is the
+// rest population the reference updates in place).  cte1 = 1 for shifted
+// storage (SRC/col_MC.F90:86-91).
+template <typename C>
+__device__ __forceinline__ void collide_site(const C (&x)[19], C (&b)[19], C omega, C cte1) {
+    using k = K<C>;
+    const C cte0 = k::uno - cte1;
+    const C x01 = x[0], x02 = x[1], x03 = x[2], x04 = x[3], x05 = x[4], x06 = x[5], x07 = x[6];
+    const C x08 = x[7], x09 = x[8], x10 = x[9], x11 = x[10], x12 = x[11], x13 = x[12];
+    const C x14 = x[13], x15 = x[14], x16 = x[15], x17 = x[16], x18 = x[17], x19 = x[18];
+
+    const C rho = x01 + x02 + x03 + x04 + x05 + x06 + x07 + x08 + x09 + x10 + x11 + x12 + x13 +
+                  x14 + x15 + x16 + x17 + x18 + x19 + cte1;
+    const C rhoinv = k::uno / rho;
+    C vx = (x01 + x02 + x03 + x04 + x05 - x10 - x11 - x12 - x13 - x14) * rhoinv;
+    C vy = (x03 + x07 + x08 + x09 + x12 - x01 - x10 - x16 - x17 - x18) * rhoinv;
+    C vz = (x04 + x06 + x07 + x13 + x18 - x02 - x09 - x11 - x15 - x16) * rhoinv;
+
+    const C vx2 = vx * vx, vy2 = vy * vy, vz2 = vz * vz;
+    const C vsq = vx2 + vy2 + vz2;
+    C vxpy = vx + vy;
+    const C qxpy = cte0 + k::qf * (k::tre * vxpy * vxpy - vsq);
+    C vxmy = vx - vy;
+    const C qxmy = cte0 + k::qf * (k::tre * vxmy * vxmy - vsq);
+    C vxpz = vx + vz;
+    const C qxpz = cte0 + k::qf * (k::tre * vxpz * vxpz - vsq);
+    C vxmz = vx - vz;
+    const C qxmz = cte0 + k::qf * (k::tre * vxmz * vxmz - vsq);
+    C vypz = vy + vz;
+    const C qypz = cte0 + k::qf * (k::tre * vypz * vypz - vsq);
+    C vymz = vy - vz;
+    const C qymz = cte0 + k::qf * (k::tre * vymz * vymz - vsq);
+    const C qx = cte0 + k::qf * (k::tre * vx2 - vsq);
+    const C qy = cte0 + k::qf * (k::tre * vy2 - vsq);
+    const C qz = cte0 + k::qf * (k::tre * vz2 - vsq);
+    const C q0 = cte0 + k::qf * (-vsq);
+
+    vx = k::rf * vx;
+    vy = k::rf * vy;
+    vz = k::rf * vz;
+    vxpy = k::rf * vxpy;
+    vxmy = k::rf * vxmy;
+    vxpz = k::rf * vxpz;
+    vxmz = k::rf * vxmz;
+    vypz = k::rf * vypz;
+    vymz = k::rf * vymz;
+
+    const C rp0 = rho * k::p0, rp1 = rho * k::p1, rp2 = rho * k::p2;
+    const C s2 = cte1 * (rp2 - k::p2), s1 = cte1 * (rp1 - k::p1), s0 = cte1 * (rp0 - k::p0);
+    const C e01 = rp2 * (+vxmy + qxmy) + s2;
+    const C e02 = rp2 * (+vxmz + qxmz) + s2;
+    const C e03 = rp2 * (+vxpy + qxpy) + s2;
+    const C e04 = rp2 * (+vxpz + qxpz) + s2;
+    const C e05 = rp1 * (+vx + qx) + s1;
+    const C e06 = rp1 * (+vz + qz) + s1;
+    const C e07 = rp2 * (+vypz + qypz) + s2;
+    const C e08 = rp1 * (+vy + qy) + s1;
+    const C e09 = rp2 * (+vymz + qymz) + s2;
+    const C e10 = rp2 * (-vxpy + qxpy) + s2;
+    const C e11 = rp2 * (-vxpz + qxpz) + s2;
+    const C e12 = rp2 * (-vxmy + qxmy) + s2;
+    const C e13 = rp2 * (-vxmz + qxmz) + s2;
+    const C e14 = rp1 * (-vx + qx) + s1;
+    const C e15 = rp1 * (-vz + qz) + s1;
+    const C e16 = rp2 * (-vypz + qypz) + s2;
+    const C e17 = rp1 * (-vy + qy) + s1;
+    const C e18 = rp2 * (-vymz + qymz) + s2;
+    const C e19 = rp0 * (+q0) + s0;
+
+    b[0] = x01 - omega * (x01 - e01);
+    b[1] = x02 - omega * (x02 - e02);
+    b[2] = x03 - omega * (x03 - e03);
+    b[3] = x04 - omega * (x04 - e04);
+    b[4] = x05 - omega * (x05 - e05);
+    b[5] = x06 - omega * (x06 - e06);
+    b[6] = x07 - omega * (x07 - e07);
+    b[7] = x08 - omega * (x08 - e08);
+    b[8] = x09 - omega * (x09 - e09);
+    b[9] = x10 - omega * (x10 - e10);
+    b[10] = x11 - omega * (x11 - e11);
+    b[11] = x12 - omega * (x12 - e12);
+    b[12] = x13 - omega * (x13 - e13);
+    b[13] = x14 - omega * (x14 - e14);
+    b[14] = x15 - omega * (x15 - e15);
+    b[15] = x16 - omega * (x16 - e16);
+    b[16] = x17 - omega * (x17 - e17);
+    b[17] = x18 - omega * (x18 - e18);
+    b[18] = x19 - omega * (x19 - e19);
+}
+
+// At-site moments of SRC/diagno.F90:96-111 (also the first lines of
+// SRC/dissipation.F90:97-114): returns rho, u and |u|^2/2 in the compute kind.
+template <typename C>
+__device__ __forceinline__ void moments_site(const C (&x)[19], C cte1, C &rho, C &ux, C &uy, C &uz) {
+    const C x01 = x[0], x02 = x[1], x03 = x[2], x04 = x[3], x05 = x[4], x06 = x[5], x07 = x[6];
+    const C x08 = x[7], x09 = x[8], x10 = x[9], x11 = x[10], x12 = x[11], x13 = x[12];
+    const C x14 = x[13], x15 = x[14], x16 = x[15], x17 = x[16], x18 = x[17], x19 = x[18];
+    rho = x01 + x02 + x03 + x04 + x05 + x06 + x07 + x08 + x09 + x10 + x11 + x12 + x13 + x14 +
+          x15 + x16 + x17 + x18 + x19 + cte1;
+    const C rhoinv = K<C>::uno / rho;
+    ux = (x01 + x02 + x03 + x04 + x05 - x10 - x11 - x12 - x13 - x14) * rhoinv;
+    uy = (x03 + x07 + x08 + x09 + x12 - x01 - x10 - x16 - x17 - x18) * rhoinv;
+    uz = (x04 + x06 + x07 + x13 + x18 - x02 - x09 - x11 - x15 - x16) * rhoinv;
+}
+
+// One site of SRC/dissipation.F90:97-249: returns 2 svisc P (P the squared
+// Frobenius norm of Pi^neq) and |u|^2/2, both in the compute kind.
+template <typename C>
+__device__ __forceinline__ void dissipation_site(const C (&x)[19], C cte1, C svisc, C &dsite,
+                                                 C &tsite) {
+    using k = K<C>;
+    const C cte0 = k::uno - cte1;
+    const C x01 = x[0], x02 = x[1], x03 = x[2], x04 = x[3], x05 = x[4], x06 = x[5], x07 = x[6];
+    const C x08 = x[7], x09 = x[8], x10 = x[9], x11 = x[10], x12 = x[11], x13 = x[12];
+    const C x14 = x[13], x15 = x[14], x16 = x[15], x17 = x[16], x18 = x[17], x19 = x[18];
+    const C rho = x01 + x02 + x03 + x04 + x05 + x06 + x07 + x08 + x09 + x10 + x11 + x12 + x13 +
+                  x14 + x15 + x16 + x17 + x18 + x19 + cte1;
+    const C rhoinv = k::uno / rho;
+    C vx = (x01 + x02 + x03 + x04 + x05 - x10 - x11 - x12 - x13 - x14) * rhoinv;
+    C vy = (x03 + x07 + x08 + x09 + x12 - x01 - x10 - x16 - x17 - x18) * rhoinv;
+    C vz = (x04 + x06 + x07 + x13 + x18 - x02 - x09 - x11 - x15 - x16) * rhoinv;
+    const C vx2 = vx * vx, vy2 = vy * vy, vz2 = vz * vz;
+    const C vsq = vx2 + vy2 + vz2;
+    tsite = (C)0.5 * vsq;
+    C vxpy = vx + vy;
+    const C qxpy = cte0 + k::qf * (k::tre * vxpy * vxpy - vsq);
+    C vxmy = vx - vy;
+    const C qxmy = cte0 + k::qf * (k::tre * vxmy * vxmy - vsq);
+    C vxpz = vx + vz;
+    const C qxpz = cte0 + k::qf * (k::tre * vxpz * vxpz - vsq);
+    C vxmz = vx - vz;
+    const C qxmz = cte0 + k::qf * (k::tre * vxmz * vxmz - vsq);
+    C vypz = vy + vz;
+    const C qypz = cte0 + k::qf * (k::tre * vypz * vypz - vsq);
+    C vymz = vy - vz;
+    const C qymz = cte0 + k::qf * (k::tre * vymz * vymz - vsq);
+    const C qx = cte0 + k::qf * (k::tre * vx2 - vsq);
+    const C qy = cte0 + k::qf * (k::tre * vy2 - vsq);
+    const C qz = cte0 + k::qf * (k::tre * vz2 - vsq);
+    vx = k::rf * vx;
+    vy = k::rf * vy;
+    vz = k::rf * vz;
+    vxpy = k::rf * vxpy;
+    vxmy = k::rf * vxmy;
+    vxpz = k::rf * vxpz;
+    vxmz = k::rf * vxmz;
+    vypz = k::rf * vypz;
+    vymz = k::rf * vymz;
+    const C rp1 = rho * k::p1, rp2 = rho * k::p2;
+    const C s2 = cte1 * (rp2 - k::p2), s1 = cte1 * (rp1 - k::p1);
+    const C n01 = x01 - (rp2 * (+vxmy + qxmy) + s2);
+    const C n02 = x02 - (rp2 * (+vxmz + qxmz) + s2);
+    const C n03 = x03 - (rp2 * (+vxpy + qxpy) + s2);
+    const C n04 = x04 - (rp2 * (+vxpz + qxpz) + s2);
+    const C n05 = x05 - (rp1 * (+vx + qx) + s1);
+    const C n06 = x06 - (rp1 * (+vz + qz) + s1);
+    const C n07 = x07 - (rp2 * (+vypz + qypz) + s2);
+    const C n08 = x08 - (rp1 * (+vy + qy) + s1);
+    const C n09 = x09 - (rp2 * (+vymz + qymz) + s2);
+    const C n10 = x10 - (rp2 * (-vxpy + qxpy) + s2);
+    const C n11 = x11 - (rp2 * (-vxpz + qxpz) + s2);
+    const C n12 = x12 - (rp2 * (-vxmy + qxmy) + s2);
+    const C n13 = x13 - (rp2 * (-vxmz + qxmz) + s2);
+    const C n14 = x14 - (rp1 * (-vx + qx) + s1);
+    const C n15 = x15 - (rp1 * (-vz + qz) + s1);
+    const C n16 = x16 - (rp2 * (-vypz + qypz) + s2);
+    const C n17 = x17 - (rp1 * (-vy + qy) + s1);
+    const C n18 = x18 - (rp2 * (-vymz + qymz) + s2);
+    const C Pxx = n01 + n02 + n03 + n04 + n05 + n10 + n11 + n12 + n13 + n14;
+    const C Pyy = n01 + n03 + n07 + n08 + n09 + n10 + n12 + n16 + n17 + n18;
+    const C Pzz = n02 + n04 + n06 + n07 + n09 + n11 + n13 + n15 + n16 + n18;
+    const C Pxz = -n02 + n04 + n11 - n13;
+    const C Pxy = -n01 + n03 + n10 - n12;
+    const C Pyz = +n07 - n09 + n16 - n18;
+    const C two = (C)2.0;
+    const C Ptotal = Pxx * Pxx + Pyy * Pyy + Pzz * Pzz + (two * Pxy * Pxy) + (two * Pxz * Pxz) +
+                     (two * Pyz * Pyz);
+    dsite = two * svisc * Ptotal;
+}
+
+}  // namespace bgk

@@ -1,0 +1,179 @@
+/*
+ * bgk3d_hit.h -- C-ABI of the MI355X (gfx950) BGK3D_HIT time-step hot path.
+ *
+ * The reference (gamati01/BGK3D_HIT, Fortran + OpenACC) has no FFI: its hot
+ * path is a set of external subroutines that share module globals
+ * (SRC/storage.F90).  Each entry point below replaces one of them; the
+ * Fortran binding is bgk3d_hit_amd/fortran/bgk_capi.F90 and the Python
+ * binding is bgk3d_hit_amd/_lib.py (ctypes).
+ *
+ * Conventions
+ *   - every function returns an int status (BGK_OK == 0); the message of the
+ *     last failure on the calling thread is read with bgk_last_error.
+ *   - host arrays are plain C arrays of doubles: populations are 19 planes of
+ *     lx*ly*lz values (x fastest, then y, then z; reference population order
+ *     01..19), velocity fields are 3 planes (ux, uy, uz) of lx*ly*lz values.
+ *     Population values are what the reference stores: f_i - w_i unless the
+ *     context was created with BGK_FLAG_NOSHIFT.
+ *   - device state stays resident; the reference's a/b pointer swap
+ *     (SRC/col_MC.F90:365-421) becomes a ping-pong index inside the context.
+ *   - a context is used by one host thread at a time.
+ */
+#ifndef BGK3D_HIT_H
+#define BGK3D_HIT_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct bgk_ctx bgk_ctx;
+
+/* status codes */
+#define BGK_OK 0
+#define BGK_EINVAL 1     /* invalid argument (size, kind, omega, ...)          */
+#define BGK_EHIP 2       /* HIP runtime error                                  */
+#define BGK_ENOMEM 3     /* device or host allocation failed                   */
+#define BGK_ESTATE 4     /* call not valid in the context's current state      */
+#define BGK_ENONFINITE 5 /* a reduction found a NaN/Inf (diverged populations) */
+#define BGK_ENODEV 6     /* no usable GPU                                      */
+
+/* storage kinds (reference SRC/precision.F90:13-32 for the first three) */
+#define BGK_STORE_F64 0       /* DOUBLE_P: store f64, compute f64                 */
+#define BGK_STORE_F32 1       /* default : store f32, compute f32                 */
+#define BGK_STORE_F32_MIXED 2 /* MIXEDPRECISION: store f32, compute f64           */
+#define BGK_STORE_F16 3       /* store shifted f_i - w_i as IEEE half, compute f32 */
+#define BGK_STORE_BF16 4      /* store shifted f_i - w_i as bfloat16, compute f32  */
+
+/* layouts */
+#define BGK_LAYOUT_PERIODIC 0 /* triply periodic, wrap in the index math          */
+#define BGK_LAYOUT_SLAB_Z 1   /* one rank's slab: x,y periodic, 1-site ghost planes
+                                 on -z/+z filled by pack / exchange / unpack       */
+
+/* creation flags */
+#define BGK_FLAG_NOSHIFT 1         /* reference -DNOSHIFT: store f_i, not f_i - w_i   */
+#define BGK_FLAG_REFCOMPAT_HALOS 2 /* keep the stale-halo images the reference's
+                                      dissipation reads (periodic layout)          */
+#define BGK_FLAG_TIMING 4          /* record HIP events around every kernel launch  */
+
+/* diagnostic modes */
+#define BGK_DIAG_CONSISTENT 0 /* gathers read the current state (fresh periodic images) */
+#define BGK_DIAG_REFCOMPAT 1  /* reproduce the reference: dissipation reads the halo
+                                 images left by the exchange two steps earlier
+                                 (SRC/dissipation.F90:76-94), diagno's 4th column is
+                                 ytot + zj(l,m,n) (SRC/diagno.F90:110)           */
+
+/* library version, e.g. 100 for 0.1.0 */
+int bgk_version(void);
+
+/* copy the calling thread's last error message into buf (NUL-terminated) */
+int bgk_last_error(char *buf, int len);
+
+/* number of visible GPUs (0 when none); does not create a context */
+int bgk_device_count(int *count);
+
+/* Allocate the device lattice: replaces alloca (SRC/alloca.F90:178-216) and
+ * the relaxation setup of hencol (SRC/hencol.F90:31, omega passed directly).
+ * Populations start at zero (the rest state when shifted).                 */
+int bgk_create(int lx, int ly, int lz, int storage, int layout, int flags, double omega,
+               bgk_ctx **out);
+int bgk_destroy(bgk_ctx *ctx);
+
+/* change the relaxation frequency (SRC/hencol.F90:31) */
+int bgk_set_omega(bgk_ctx *ctx, double omega);
+
+/* width of the boundary region used by bgk_col_MC_masked in the periodic
+ * layout (reference namelist key `border`, SRC/input.F90:53, mask built in
+ * SRC/init.F90:44-62); default 10                                          */
+int bgk_set_border(bgk_ctx *ctx, int border);
+
+/* info[0..7] = lx, ly, lz, storage, layout, flags, device bytes, steps done */
+int bgk_info(bgk_ctx *ctx, int64_t info[8]);
+
+/* upload / download the current populations (19 planes, see conventions);
+ * replaces the host-array view the reference's init and I/O use
+ * (SRC/init.F90:119-137, SRC/bgk3d_HIT.F90:111-115 data region)          */
+int bgk_set_populations(bgk_ctx *ctx, const double *f);
+int bgk_get_populations(bgk_ctx *ctx, double *f);
+
+/* population initialisation from a velocity field with rho = 1: the
+ * equilibrium formula of init (SRC/init.F90:97-137, crho = 1)             */
+int bgk_init_equilibrium(bgk_ctx *ctx, const double *u);
+
+/* Seeded homogeneous-isotropic-turbulence velocity field (not in the
+ * reference, whose init is a Taylor-Green vortex, SRC/init.F90:83-92):
+ * solenoidal random-phase Fourier field, E(k) ~ k^4 exp(-2 (k/k0)^2),
+ * zero mean, rescaled so that sqrt(<|u|^2>/3) = urms.  Host computation into
+ * u (3 planes).  bgk_init_hit synthesises the same field on the GPU and
+ * applies bgk_init_equilibrium.                                            */
+int bgk_hit_velocity(int lx, int ly, int lz, double k0, double urms, uint64_t seed, double *u);
+int bgk_init_hit(bgk_ctx *ctx, double k0, double urms, uint64_t seed);
+
+/* Halo path: replaces boundaries (SRC/boundaries.F90:23-65) and its default
+ * bcond_comm_step3 (SRC/bcond.comm.step3.F90:29-316).  Slab layout: pack the
+ * 5 crossing populations of each z face, exchange (periodic self-exchange
+ * of the packed messages), unpack into the ghost planes.  Periodic layout:
+ * nothing to exchange (wrap is in the index math); with
+ * BGK_FLAG_REFCOMPAT_HALOS the face images are recorded as the reference's
+ * ghosts would hold them.                                                   */
+int bgk_boundaries(bgk_ctx *ctx);
+
+/* fused pull stream + BGK collide over all sites, then swap: replaces col_MC
+ * (SRC/col_MC.F90:34-444)                                                   */
+int bgk_col_MC(bgk_ctx *ctx);
+
+/* collide restricted to the bulk (opt = 0) or the boundary region (opt = 1),
+ * swap only after opt = 1: replaces col_MC_masked (SRC/col_MC_masked.F90:34-451)
+ * without a mask array.  Slab layout: the boundary region is the two planes
+ * that read ghosts; periodic layout: sites within `border` of a face.      */
+int bgk_col_MC_masked(bgk_ctx *ctx, int opt);
+
+/* nsteps of the time loop body boundaries + col_MC (SRC/bgk3d_HIT.F90:127-146
+ * without diagnostic).  Slab layout: the interior collide overlaps the halo
+ * pack/exchange/unpack and boundary collide on a second HIP stream (the
+ * structure of SRC/bcond.comm.step9.F90:349 and
+ * SRC/do_somethingGPU_overlap.F90:48-58).                                   */
+int bgk_step(bgk_ctx *ctx, int nsteps);
+
+/* at-site moments of the current state: replaces diagno (SRC/diagno.F90:28-154).
+ * out = mean rho, mean ux, mean uy, mean uz, mean |u|^2/2 (the reference's
+ * loctot(1..5)), FP64 accumulation.  Returns BGK_ENONFINITE on NaN/Inf.     */
+int bgk_diagno(bgk_ctx *ctx, int mode, double out[5]);
+
+/* pulled-moment dissipation and kinetic energy of the current state:
+ * replaces dissipation (SRC/dissipation.F90:22-280).  out[0] = sum over
+ * sites of 2 svisc Pi^neq:Pi^neq, out[1] = sum over sites of |u|^2/2 (the
+ * reference's loctot(1..2), before its output normalisation).              */
+int bgk_dissipation(bgk_ctx *ctx, double svisc, int mode, double out[2]);
+
+/* wait for all device work of the context */
+int bgk_synchronize(bgk_ctx *ctx);
+
+/* With BGK_FLAG_TIMING: t[0] = total ms in collide kernels, t[1] = collide
+ * launches, t[2] = ms in reduction kernels, t[3] = reduction launches,
+ * t[4] = ms in halo kernels/copies, t[5] = halo launches, t[6] = ms of
+ * bgk_step wall time on the device (first to last event).  reset != 0
+ * clears the counters after reading.  Replaces the reference's section
+ * timers (SRC/mod_tools.F90:11-54, SRC/finalize.F90:44-97).                 */
+int bgk_timing(bgk_ctx *ctx, double t[8], int reset);
+
+/* Discrete adjoint (not in the reference).  J = sum_x |u(x)|^2 / 2 of the
+ * state reached after nsteps steps from the equilibrium of u0 (rho = 1);
+ * grad = dJ/du0 (3 planes).  Checkpoints every ckpt_every steps with
+ * recompute; levels = 1 or 2 (two-level checkpointing).  stats[0] = peak
+ * device bytes of the adjoint's buffers, stats[1] = forward ms,
+ * stats[2] = backward ms, stats[3] = number of recomputed steps.
+ * u0 and grad are host arrays.                                              */
+int bgk_adjoint_ke(bgk_ctx *ctx, const double *u0, int nsteps, int ckpt_every, int levels,
+                   double *J, double *grad, double stats[4]);
+/* Same with u0 and grad in device memory (for framework bindings). */
+int bgk_adjoint_ke_device(bgk_ctx *ctx, const double *u0_dev, int nsteps, int ckpt_every,
+                          int levels, double *J, double *grad_dev, double stats[4]);
+/* Forward functional only: J after nsteps from the equilibrium of u0 (host). */
+int bgk_forward_ke(bgk_ctx *ctx, const double *u0, int nsteps, double *J);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,233 @@
+"""GPU parity: the HIP library (through the C-ABI) against the CPU oracle.
+
+Bar: FP64/FP32/F32-mixed/F16/BF16 populations bit-identical to the oracle's
+restatement of the reference arithmetic (same IEEE operations, no FMA
+contraction on either side); diagnostics within 1e-10 relative in FP64 (the
+GPU sums in a tree, the oracle serially).  At full sizes the checks are
+size-independent properties (conservation, slab/periodic identity).
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import bgk3d_hit_amd as bgk
+import oracle
+from tests import d3q19_numpy as ref
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print("[gpu-case]", *a, flush=True)
+    sys.stdout.flush()
+
+
+def rel(a, b):
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+
+
+def hit_pair(shape, k0, urms, seed, omega, kind="f64", refcompat=False):
+    u = bgk.hit_velocity(*shape, k0, urms, seed)
+    g = bgk.Lattice(*shape, omega=omega, storage=kind, refcompat_halos=refcompat)
+    o = oracle.OracleLattice(*shape, omega=omega, kind=kind)
+    g.init_equilibrium(u)
+    o.init_equilibrium(u)
+    return g, o, u
+
+
+@pytest.mark.parametrize("kind", ["f64", "f32", "f32mix", "f16", "bf16"])
+def test_init_equilibrium_bitwise(gpu, kind):
+    shape = (12, 10, 8)
+    say("init_equilibrium", kind, shape)
+    g, o, _ = hit_pair(shape, 2.0, 0.03, 7, 1.9, kind)
+    assert np.array_equal(g.get_populations(), o.get_populations())
+
+
+@pytest.mark.parametrize("kind", ["f64", "f32", "f32mix", "f16", "bf16"])
+def test_one_step_bitwise(gpu, kind):
+    shape = (70, 9, 6)  # x not a multiple of the 64-wide tile
+    say("one step", kind, shape)
+    rng = np.random.default_rng(1)
+    f0 = 2e-3 * rng.standard_normal((19, shape[2], shape[1], shape[0]))
+    g = bgk.Lattice(*shape, omega=1.93, storage=kind)
+    o = oracle.OracleLattice(*shape, omega=1.93, kind=kind)
+    g.set_populations(f0)
+    o.set_populations(f0)
+    assert np.array_equal(g.get_populations(), o.get_populations())
+    g.step(1)
+    o.step(1)
+    assert np.array_equal(g.get_populations(), o.get_populations())
+
+
+def test_pure_streaming_shift(gpu):
+    """omega = 0: exact periodic shift by c_q per step
+    (SRC/do_somethingGPU.F90:63-73 semantics)."""
+    shape = (65, 7, 5)
+    say("pure streaming", shape)
+    rng = np.random.default_rng(2)
+    f0 = rng.standard_normal((19, 5, 7, 65))
+    g = bgk.Lattice(*shape, omega=0.0)
+    g.set_populations(f0)
+    g.step(4)
+    exp = f0
+    for _ in range(4):
+        exp = ref.pull(exp)
+    assert np.array_equal(g.get_populations(), exp)
+
+
+@pytest.mark.parametrize("kind,nsteps", [("f64", 200), ("f32", 200), ("f32mix", 100),
+                                         ("f16", 100), ("bf16", 100)])
+def test_hit_trajectory_bitwise(gpu, kind, nsteps):
+    shape = (32, 32, 32)
+    say("trajectory", kind, shape, nsteps)
+    g, o, _ = hit_pair(shape, 4.0, 0.02, 1234, 1.90, kind)
+    g.step(nsteps)
+    o.step(nsteps)
+    assert np.array_equal(g.get_populations(), o.get_populations())
+
+
+def test_config1_64cubed_1000_steps_with_diagnostics(gpu):
+    """64^3 HIT, k0=4, urms=0.02, omega=1.90, FP64, 1000 steps, diagnostics
+    every 100 steps in the reference loop order."""
+    shape = (64, 64, 64)
+    omega = 1.90
+    nu = bgk.svisc_from_omega(omega)
+    say("64^3 x 1000 FP64")
+    g, o, _ = hit_pair(shape, 4.0, 0.02, 1234, omega, "f64", refcompat=True)
+    for it in range(1, 1001):
+        g.boundaries()
+        g.col_MC()
+        o.bcond_step3()
+        o.col_MC()
+        if it % 100 == 0:
+            dg, do = g.diagno(), o.diagno()
+            assert abs(dg[0] - do[0]) < 1e-14
+            assert np.abs(dg[1:4] - do[1:4]).max() < 1e-10 * 0.02
+            assert abs(dg[4] - do[4]) < 1e-10 * abs(do[4])
+            # reference-compatible dissipation (stale halos), then consistent
+            sg, so = g.dissipation(nu, refcompat=True), o.dissipation(nu)
+            assert np.abs(sg - so).max() < 1e-10 * np.abs(so).max(), (it, sg, so)
+            bg = g.diagno(refcompat=True)
+            bo = o.diagno(ztot_bug=True)
+            assert abs(bg[3] - bo[3]) < 1e-10 * 0.02
+            o.bcond_step3()
+            cg, co = g.dissipation(nu), o.dissipation(nu)
+            assert np.abs(cg - co).max() < 1e-10 * np.abs(co).max()
+    fg, fo = g.get_populations(), o.get_populations()
+    assert rel(fg, fo) <= 1e-12
+    assert np.array_equal(fg, fo)
+
+
+@pytest.mark.parametrize("kind", ["f64", "f32"])
+def test_masked_bulk_then_border_equals_full(gpu, kind):
+    shape = (40, 36, 30)
+    say("masked", kind)
+    g1, _, u = hit_pair(shape, 3.0, 0.03, 3, 1.95, kind)
+    g2 = bgk.Lattice(*shape, omega=1.95, storage=kind)
+    g2.init_equilibrium(u)
+    g2.set_border(5)
+    for _ in range(3):
+        g1.col_MC()
+        g2.col_MC_masked(0)
+        g2.col_MC_masked(1)
+    assert np.array_equal(g1.get_populations(), g2.get_populations())
+
+
+@pytest.mark.parametrize("kind", ["f32", "f64", "f16"])
+def test_slab_overlap_bitwise_equals_periodic(gpu, kind):
+    """One rank's slab with +-z ghost planes, pack / self-exchange / unpack,
+    bulk and boundary collides on two streams: identical to the periodic
+    kernel on the same domain."""
+    shape = (64, 48, 16)
+    say("slab", kind, shape)
+    u = bgk.hit_velocity(*shape, 4.0, 0.03, 5678)
+    p = bgk.Lattice(*shape, omega=1.97, storage=kind)
+    s = bgk.Lattice(*shape, omega=1.97, storage=kind, layout="slab")
+    s2 = bgk.Lattice(*shape, omega=1.97, storage=kind, layout="slab")
+    for x in (p, s, s2):
+        x.init_equilibrium(u)
+    p.step(25)
+    s.step(25)                       # overlapped two-stream path
+    for _ in range(25):              # reference call order, one stream
+        s2.boundaries()
+        s2.col_MC()
+    fp = p.get_populations()
+    assert np.array_equal(s.get_populations(), fp)
+    assert np.array_equal(s2.get_populations(), fp)
+    # consistent dissipation refreshes the ghosts first: same as periodic
+    nu = bgk.svisc_from_omega(1.97)
+    assert np.allclose(s.dissipation(nu), p.dissipation(nu), rtol=1e-13, atol=0)
+
+
+def test_slab_masked_path(gpu):
+    shape = (32, 32, 12)
+    say("slab masked")
+    u = bgk.hit_velocity(*shape, 2.0, 0.03, 9)
+    p = bgk.Lattice(*shape, omega=1.9, storage="f32")
+    s = bgk.Lattice(*shape, omega=1.9, storage="f32", layout="slab")
+    p.init_equilibrium(u)
+    s.init_equilibrium(u)
+    for _ in range(5):
+        p.step(1)
+        s.boundaries()
+        s.col_MC_masked(0)
+        s.col_MC_masked(1)
+    assert np.array_equal(s.get_populations(), p.get_populations())
+
+
+def test_device_hit_equals_host_hit(gpu):
+    shape = (40, 32, 24)
+    say("device HIT")
+    u = bgk.hit_velocity(*shape, 3.0, 0.025, 42)
+    a = bgk.Lattice(*shape, omega=1.9)
+    b = bgk.Lattice(*shape, omega=1.9)
+    a.init_equilibrium(u)
+    b.init_hit(3.0, 0.025, 42)
+    assert np.array_equal(a.get_populations(), b.get_populations())
+
+
+def test_nonfinite_populations_are_reported(gpu):
+    shape = (16, 16, 16)
+    say("nan injection")
+    g = bgk.Lattice(*shape, omega=1.9)
+    f = np.zeros((19, 16, 16, 16))
+    f[3, 5, 6, 7] = np.nan
+    g.set_populations(f)
+    g.step(2)
+    with pytest.raises(bgk.BGKError) as e:
+        g.diagno()
+    assert e.value.code == 5
+    with pytest.raises(bgk.BGKError):
+        g.dissipation(0.01)
+
+
+def test_refcompat_dissipation_needs_flag(gpu):
+    g = bgk.Lattice(8, 8, 8, omega=1.9)
+    with pytest.raises(bgk.BGKError) as e:
+        g.dissipation(0.01, refcompat=True)
+    assert e.value.code == 4
+
+
+def test_timing_counters(gpu):
+    g = bgk.Lattice(64, 64, 64, omega=1.9, storage="f32", timing=True)
+    g.init_hit(4.0, 0.02, 1)
+    g.step(10)
+    g.diagno()
+    t = g.timing()
+    assert t["collide_launches"] == 10 and t["reduce_launches"] == 1
+    assert t["collide_ms"] > 0 and t["step_ms"] >= t["collide_ms"] * 0.5
+
+
+def test_512_cubed_fp32_conservation(gpu):
+    """Full-size property: mass and momentum of the 512^3 FP32 HIT lattice
+    are conserved to FP32 roundoff over 20 steps."""
+    say("512^3 f32 conservation")
+    g = bgk.Lattice(512, 512, 512, omega=1.95, storage="f32")
+    g.init_hit(4.0, 0.02, 1234)
+    d0 = g.diagno()
+    g.step(20)
+    d1 = g.diagno()
+    assert abs(d1[0] - d0[0]) < 1e-6
+    assert np.abs(d1[1:4] - d0[1:4]).max() < 1e-6 * 0.02
+    assert d1[4] < d0[4]  # decaying turbulence loses kinetic energy
