@@ -107,6 +107,27 @@ def cpu_baseline(min_seconds=10.0, max_steps=8):
                       f"workload, {steps} steps in {wall:.1f} s"}
 
 
+def calibrate_copy(gib, reps=10):
+    """Plain copy of a known size: dst.copy_(src) moves gib GiB in and gib GiB out."""
+    import torch
+
+    n = int(gib * (1 << 30)) // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        dst.copy_(src)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    nbytes = 2 * n * 4
+    print(json.dumps({"copy_bytes_read": n * 4, "copy_bytes_written": n * 4,
+                      "avg_ms": ms, "gbs": nbytes / (ms * 1e-3) / 1e9}), flush=True)
+
+
 def load_traffic(kind):
     p = os.path.join(ROOT, "profiles", "collide_traffic.json")
     try:
@@ -127,7 +148,13 @@ def main():
     ap.add_argument("--variants", default="f64,f16,bf16",
                     help="extra storage kinds to measure beside f32 ('' for none)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calibrate-copy", type=float, default=0.0, metavar="GIB",
+                    help="only time a plain device copy of GIB GiB (the reference kernel "
+                         "for scaling rocprofv3 FETCH_SIZE/WRITE_SIZE) and exit")
     a = ap.parse_args()
+    if a.calibrate_copy > 0:
+        calibrate_copy(a.calibrate_copy)
+        return
     if a.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {a.gpus}); "
               "the one-GPU throughput is the whole-job value", file=sys.stderr)

@@ -75,8 +75,13 @@ struct Storage<half_t> {
     }
     template <typename C>
     __device__ __forceinline__ static half_t store(C v) {
-        // v_cvt_f16_f32: IEEE round to nearest even (default mode)
-        return half_t{__builtin_bit_cast(uint16_t, (_Float16)(float)v)};
+        // Round to f32 first, then to f16 (v_cvt_f16_f32, nearest even).  The
+        // empty asm keeps the f32 value opaque: without it the compiler fuses
+        // (half)(a*b) into v_fma_mixlo_f16, which rounds a*b once, directly
+        // to f16, and differs from rounding the f32 product.
+        float f = (float)v;
+        asm volatile("" : "+v"(f));
+        return half_t{__builtin_bit_cast(uint16_t, (_Float16)f)};
     }
 };
 template <>

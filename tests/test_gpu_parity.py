@@ -102,7 +102,7 @@ def test_config1_64cubed_1000_steps_with_diagnostics(gpu):
         o.col_MC()
         if it % 100 == 0:
             dg, do = g.diagno(), o.diagno()
-            assert abs(dg[0] - do[0]) < 1e-14
+            assert abs(dg[0] - do[0]) < 1e-12  # oracle sums 64^3 values serially
             assert np.abs(dg[1:4] - do[1:4]).max() < 1e-10 * 0.02
             assert abs(dg[4] - do[4]) < 1e-10 * abs(do[4])
             # reference-compatible dissipation (stale halos), then consistent
@@ -219,15 +219,28 @@ def test_timing_counters(gpu):
     assert t["collide_ms"] > 0 and t["step_ms"] >= t["collide_ms"] * 0.5
 
 
-def test_512_cubed_fp32_conservation(gpu):
-    """Full-size property: mass and momentum of the 512^3 FP32 HIT lattice
-    are conserved to FP32 roundoff over 20 steps."""
-    say("512^3 f32 conservation")
-    g = bgk.Lattice(512, 512, 512, omega=1.95, storage="f32")
-    g.init_hit(4.0, 0.02, 1234)
-    d0 = g.diagno()
-    g.step(20)
-    d1 = g.diagno()
-    assert abs(d1[0] - d0[0]) < 1e-6
-    assert np.abs(d1[1:4] - d0[1:4]).max() < 1e-6 * 0.02
+def test_512_cubed_fp32_against_fp64(gpu):
+    """Full-size check of the 512^3 workload: FP64 conserves mass to
+    roundoff over 20 steps, and the FP32 run (the reference's
+    single-precision arithmetic, bit-exact with the oracle at small sizes)
+    tracks it: kinetic energy within 1e-5 relative, mean density within
+    1e-6."""
+    say("512^3 f32 vs f64")
+    res = {}
+    for kind in ("f64", "f32"):
+        g = bgk.Lattice(512, 512, 512, omega=1.95, storage=kind)
+        g.init_hit(4.0, 0.02, 1234)
+        d0 = g.diagno()
+        g.step(20)
+        res[kind] = (d0, g.diagno(), g.dissipation(bgk.svisc_from_omega(1.95)))
+        g.close()
+    d0, d1, s64 = res["f64"]
+    assert abs(d1[0] - d0[0]) < 1e-13  # mass
+    # (the mean of u = j/rho is not an invariant -- sum j is -- so mean u
+    # may drift physically; it is compared between the two precisions)
     assert d1[4] < d0[4]  # decaying turbulence loses kinetic energy
+    e0, e1, s32 = res["f32"]
+    assert abs(e1[0] - d1[0]) < 1e-6
+    assert np.abs(e1[1:4] - d1[1:4]).max() < 1e-5 * 0.02
+    assert abs(e1[4] - d1[4]) < 1e-5 * d1[4]
+    assert np.abs(s32 - s64).max() < 1e-4 * np.abs(s64).max()
