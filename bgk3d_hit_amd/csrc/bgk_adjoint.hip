@@ -1,18 +1,348 @@
-// bgk_adjoint.hip -- discrete adjoint of the fused stream-collide.
-#include "bgk_internal.h"
+// bgk_adjoint.hip -- discrete adjoint of the fused stream-collide (not in the
+// reference): gradient of J = sum_x |u(x)|^2/2 at the final step with respect
+// to the initial velocity field.
+//
+// Forward step (periodic, pull):  g_k(x) = f_k(x - c_k),
+//   f'_i(x) = C_i(g(x)) = g_i - omega (g_i - e_i(g)),
+//   e_i = w_i [rho + 3 c_i.j + 9/2 (c_i.j)^2/rho - 3/2 |j|^2/rho] - w_i (shifted),
+//   rho = sum g + 1, j = sum c g, u = j / rho.
+// Adjoint step: lambda^t_k(x - c_k) = mu_k(x) with
+//   mu_k = (1 - omega) lambda_k + omega [A + (B - 3 S u) . c_k],
+//   A = sum_i w_i lambda_i (1 - 9/2 (c_i.u)^2 + 3/2 |u|^2),
+//   B = sum_i w_i lambda_i (3 + 9 c_i.u) c_i,   S = sum_i w_i lambda_i,
+// i.e. J^T applied per site as a rank-4 moment correction plus the
+// (1 - omega) diagonal, then pushed back along the streaming permutation
+// (every target written exactly once: no atomics).  The Jacobian depends on
+// the site's pulled-moment velocity u only, so the forward pass records u
+// (3 values per site and step, the "tape") instead of whole states.
+// Final condition: lambda^T_i = (u.c_i - |u|^2) / rho (at-site moments).
+// Initial map: f^0_i = w_i (3 c_i.u0 + 9/2 (c_i.u0)^2 - 3/2 |u0|^2), so
+//   dJ/du0 = sum_i lambda^0_i w_i (3 c_i + 9 (c_i.u0) c_i - 3 u0).
+//
+// Checkpointing: levels = 0 tapes every step of the horizon (no
+// recompute); levels = 1 keeps the state every ckpt_every steps and
+// re-runs each segment with the tape; levels = 2 adds coarse checkpoints
+// every ckpt_every*M steps (M ~ sqrt(number of segments)) and recomputes the
+// fine checkpoints of one coarse segment at a time.
+#include <math.h>
+
+#include <algorithm>
+#include <chrono>
+#include <vector>
+
+#include "bgk_collide.h"
 
 namespace bgk {
 
-int forward_ke(bgk_ctx *c, const double *u0_dev, int nsteps, double *J) {
-    (void)c; (void)u0_dev; (void)nsteps; (void)J;
-    return fail(BGK_ESTATE, "forward_ke: not available in this build");
+// lambda^T from at-site moments of the final state
+template <typename S, typename C>
+__global__ __launch_bounds__(256) void k_lambda_final(Pops<S> P, C *lam, long long nsite) {
+    const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (i >= nsite) return;
+    C f[19];
+#pragma unroll
+    for (int q = 0; q < 19; q++) f[q] = Storage<S>::template load<C>(P.p[q][i]);
+    C rho, ux, uy, uz;
+    moments_site<C>(f, (C)1, rho, ux, uy, uz);
+    const C usq = ux * ux + uy * uy + uz * uz;
+#pragma unroll
+    for (int q = 0; q < 19; q++)
+        lam[(size_t)q * nsite + i] = (ux * (C)cx(q) + uy * (C)cy(q) + uz * (C)cz(q) - usq) / rho;
 }
 
-int adjoint_ke(bgk_ctx *c, const double *u0_dev, int nsteps, int ckpt_every, int levels,
-               double *J, double *grad_dev, double stats[4]) {
-    (void)c; (void)u0_dev; (void)nsteps; (void)ckpt_every; (void)levels; (void)J;
-    (void)grad_dev; (void)stats;
-    return fail(BGK_ESTATE, "adjoint_ke: not available in this build");
+// one adjoint step: reads lambda^{t+1} at site x and the tape u^t(x),
+// writes lambda^t_k at x - c_k
+template <typename C>
+__global__ __launch_bounds__(256) void k_adjoint_step(const C *__restrict__ lin,
+                                                      C *__restrict__ lout,
+                                                      const C *__restrict__ tape, int nx, int ny,
+                                                      int nz, int tiles_x, int tiles_y, C omega) {
+    const int t = (int)blockIdx.x;
+    const int tx = t % tiles_x;
+    const int r = t / tiles_x;
+    const int ty = r % tiles_y;
+    const int z = r / tiles_y;
+    const int x = tx * 64 + (int)threadIdx.x;
+    const int y = ty * 4 + (int)threadIdx.y;
+    if (x >= nx || y >= ny) return;
+    const long long N = (long long)nx * ny * nz;
+    const int nxy = nx * ny;
+    const int c = x + nx * y + nxy * z;
+    const C ux = tape[c], uy = tape[N + c], uz = tape[2 * N + c];
+    const C usq = ux * ux + uy * uy + uz * uz;
+    C lam[19];
+#pragma unroll
+    for (int q = 0; q < 19; q++) lam[q] = lin[(size_t)q * N + c];
+    C A = 0, Bx = 0, By = 0, Bz = 0, Sw = 0;
+#pragma unroll
+    for (int q = 0; q < 19; q++) {
+        const C wl = (C)wq(q) * lam[q];
+        const C cu = (C)cx(q) * ux + (C)cy(q) * uy + (C)cz(q) * uz;
+        A += wl * ((C)1 - (C)4.5 * cu * cu + (C)1.5 * usq);
+        const C s = wl * ((C)3 + (C)9 * cu);
+        Bx += s * (C)cx(q);
+        By += s * (C)cy(q);
+        Bz += s * (C)cz(q);
+        Sw += wl;
+    }
+    const C Gx = Bx - (C)3 * Sw * ux, Gy = By - (C)3 * Sw * uy, Gz = Bz - (C)3 * Sw * uz;
+    const int xm = (x == 0) ? nx - 1 : x - 1, xp = (x == nx - 1) ? 0 : x + 1;
+    const int ym = (y == 0) ? ny - 1 : y - 1, yp = (y == ny - 1) ? 0 : y + 1;
+    const int zm = (z == 0) ? nz - 1 : z - 1, zp = (z == nz - 1) ? 0 : z + 1;
+    const int X[3] = {xm, x, xp};
+    const int Y[3] = {ym * nx, y * nx, yp * nx};
+    const int Z[3] = {zm * nxy, z * nxy, zp * nxy};
+#pragma unroll
+    for (int q = 0; q < 19; q++) {
+        const C mu = ((C)1 - omega) * lam[q] +
+                     omega * (A + Gx * (C)cx(q) + Gy * (C)cy(q) + Gz * (C)cz(q));
+        // population q at x came from x - c_q
+        lout[(size_t)q * N + X[1 - cx(q)] + Y[1 - cy(q)] + Z[1 - cz(q)]] = mu;
+    }
+}
+
+// dJ/du0 = sum_i lambda^0_i w_i (3 c_i + 9 (c_i.u0) c_i - 3 u0)
+template <typename C>
+__global__ __launch_bounds__(256) void k_grad_u0(const C *lam, const double *u0, double *grad,
+                                                 long long N) {
+    const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (i >= N) return;
+    const double ux = u0[i], uy = u0[N + i], uz = u0[2 * N + i];
+    double gx = 0, gy = 0, gz = 0;
+#pragma unroll
+    for (int q = 0; q < 19; q++) {
+        const double L = (double)lam[(size_t)q * N + i] * wq(q);
+        const double cu = cx(q) * ux + cy(q) * uy + cz(q) * uz;
+        gx += L * (3.0 * cx(q) + 9.0 * cu * cx(q) - 3.0 * ux);
+        gy += L * (3.0 * cy(q) + 9.0 * cu * cy(q) - 3.0 * uy);
+        gz += L * (3.0 * cz(q) + 9.0 * cu * cz(q) - 3.0 * uz);
+    }
+    grad[i] = gx;
+    grad[N + i] = gy;
+    grad[2 * N + i] = gz;
+}
+
+// ------------------------------------------------------------ host driver
+namespace {
+
+struct Adj {
+    bgk_ctx *c;
+    size_t pbytes;      // bytes of one stored population plane
+    size_t state_bytes; // 19 planes
+    size_t csize;       // sizeof compute kind
+    std::vector<void *> allocs;
+    size_t bytes = 0;
+    long long recomputed = 0;
+
+    ~Adj() {
+        for (void *p : allocs) (void)hipFree(p);
+    }
+    int alloc(void **p, size_t n, const char *what) {
+        hipError_t e = hipMalloc(p, n);
+        if (e != hipSuccess) return hip_fail(e, what);
+        allocs.push_back(*p);
+        bytes += n;
+        return BGK_OK;
+    }
+    // copy the current state (copy c->cur) to / from a 19-plane slot
+    int save(void *slot) {
+        const Lattice &L = c->lat;
+        BGK_HIP(hipMemcpyAsync(slot, L.pop[c->cur][0], pbytes * 18, hipMemcpyDeviceToDevice,
+                               c->s0));
+        BGK_HIP(hipMemcpyAsync((char *)slot + pbytes * 18, L.pop[c->cur][18], pbytes,
+                               hipMemcpyDeviceToDevice, c->s0));
+        return BGK_OK;
+    }
+    int restore(const void *slot) {
+        const Lattice &L = c->lat;
+        BGK_HIP(hipMemcpyAsync(L.pop[c->cur][0], slot, pbytes * 18, hipMemcpyDeviceToDevice,
+                               c->s0));
+        BGK_HIP(hipMemcpyAsync(L.pop[c->cur][18], (const char *)slot + pbytes * 18, pbytes,
+                               hipMemcpyDeviceToDevice, c->s0));
+        return BGK_OK;
+    }
+};
+
+template <typename S, typename C>
+hipError_t launch_tape_step(bgk_ctx *c, C *tape) {
+    const Lattice &L = c->lat;
+    int ntiles;
+    Geo g = make_geo(L, Box{0, L.nx, 0, L.ny, 0, L.nz}, &ntiles);
+    hipLaunchKernelGGL((k_collide<S, C, true>), dim3(ntiles), dim3(64, 4), 0, c->s0,
+                       pops_of<S>(L, c->cur), pops_of<S>(L, 1 - c->cur), g, (C)c->omega, (C)1,
+                       tape, L.nsite);
+    c->cur ^= 1;
+    return hipGetLastError();
+}
+
+template <typename C>
+hipError_t launch_adjoint_step(bgk_ctx *c, const C *lin, C *lout, const C *tape) {
+    const Lattice &L = c->lat;
+    const int tx = (L.nx + 63) / 64, ty = (L.ny + 3) / 4;
+    hipLaunchKernelGGL(k_adjoint_step<C>, dim3(tx * ty * L.nz), dim3(64, 4), 0, c->s0, lin, lout,
+                       tape, L.nx, L.ny, L.nz, tx, ty, (C)c->omega);
+    return hipGetLastError();
+}
+
+double ms_since(std::chrono::steady_clock::time_point t0) {
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0)
+        .count();
+}
+
+template <typename S, typename C>
+int adjoint_t(bgk_ctx *c, const double *u0, int T, int K, int levels, double *J, double *grad,
+              double stats[4]) {
+    const Lattice &L = c->lat;
+    const long long N = L.nsite;
+    Adj a;
+    a.c = c;
+    a.pbytes = ((size_t)L.plane_elems * sizeof(S) + 255) & ~(size_t)255;
+    a.state_bytes = a.pbytes * 19;
+    a.csize = sizeof(C);
+    if (K < 1) K = 1;
+    if (levels == 0) K = std::max(T, 1);
+    const int nseg = (T + K - 1) / K;
+    const int M = (levels == 2) ? std::max(1, (int)ceil(sqrt((double)nseg))) : 1;
+    const int K2 = K * M;                       // coarse interval
+    const int ncoarse = (T + K2 - 1) / K2;
+    const int tape_steps = std::min(K, std::max(T, 1));
+    std::vector<void *> coarse(ncoarse), fine(M > 1 ? M : 0);
+    int r;
+    for (int i = 0; i < ncoarse; i++)
+        if ((r = a.alloc(&coarse[i], a.state_bytes, "hipMalloc(checkpoint)"))) return r;
+    for (int i = 0; i < (int)fine.size(); i++)
+        if ((r = a.alloc(&fine[i], a.state_bytes, "hipMalloc(fine checkpoint)"))) return r;
+    C *tape = nullptr, *lamA = nullptr, *lamB = nullptr;
+    if ((r = a.alloc((void **)&tape, sizeof(C) * 3 * (size_t)N * tape_steps, "hipMalloc(tape)")))
+        return r;
+    if ((r = a.alloc((void **)&lamA, sizeof(C) * 19 * (size_t)N, "hipMalloc(lambda)"))) return r;
+    if ((r = a.alloc((void **)&lamB, sizeof(C) * 19 * (size_t)N, "hipMalloc(lambda)"))) return r;
+
+    // ---- forward: f^0 from u0, coarse checkpoints every K2 steps; the last
+    // segment's tape is recorded on the way
+    const int last_s0 = nseg > 0 ? (nseg - 1) * K : 0;
+    auto t0 = std::chrono::steady_clock::now();
+    BGK_HIP(launch_init_equilibrium(L, c->cur, u0, N, 0, N, c->s0));
+    for (int t = 0; t < T; t++) {
+        if (t % K2 == 0 && (r = a.save(coarse[t / K2]))) return r;
+        if (t >= last_s0) {
+            BGK_HIP((launch_tape_step<S, C>(c, tape + (size_t)(t - last_s0) * 3 * N)));
+        } else {
+            BGK_HIP(launch_collide(L, c->cur, Box{0, L.nx, 0, L.ny, 0, L.nz}, c->omega, c->s0));
+            c->cur ^= 1;
+        }
+    }
+    if ((r = reduce_ke(c, c->cur, J))) return r;  // synchronises
+    hipLaunchKernelGGL((k_lambda_final<S, C>), dim3((unsigned)((N + 255) / 256)), dim3(256), 0,
+                       c->s0, pops_of<S>(L, c->cur), lamA, N);
+    BGK_HIP(hipGetLastError());
+    BGK_HIP(hipStreamSynchronize(c->s0));
+    const double fwd_ms = ms_since(t0);
+
+    // ---- backward, one coarse segment at a time
+    t0 = std::chrono::steady_clock::now();
+    C *lin = lamA, *lout = lamB;
+    bool tape_is_last = true;  // the tape holds the last segment
+    for (int cs = ncoarse - 1; cs >= 0; cs--) {
+        const int c0 = cs * K2, c1 = std::min(T, c0 + K2);
+        const int nf = (c1 - c0 + K - 1) / K;
+        // fine checkpoints are needed for every segment still to be taped
+        const int need = tape_is_last ? nf - 1 : nf;
+        if (need > 0 && M > 1) {
+            if ((r = a.restore(coarse[cs]))) return r;
+            {
+                for (int j = 0; j < need; j++) {
+                    if ((r = a.save(fine[j]))) return r;
+                    if (j == need - 1) break;
+                    for (int t = 0; t < K; t++) {
+                        BGK_HIP(launch_collide(L, c->cur, Box{0, L.nx, 0, L.ny, 0, L.nz},
+                                               c->omega, c->s0));
+                        c->cur ^= 1;
+                        a.recomputed++;
+                    }
+                }
+            }
+        }
+        for (int j = nf - 1; j >= 0; j--) {
+            const int s0 = c0 + j * K, s1 = std::min(c1, s0 + K);
+            if (tape_is_last) {
+                tape_is_last = false;
+            } else {
+                if (M > 1) {
+                    if ((r = a.restore(fine[j]))) return r;
+                } else if ((r = a.restore(coarse[cs]))) {
+                    return r;
+                }
+                for (int t = s0; t < s1; t++) {
+                    BGK_HIP((launch_tape_step<S, C>(c, tape + (size_t)(t - s0) * 3 * N)));
+                    a.recomputed++;
+                }
+            }
+            for (int t = s1 - 1; t >= s0; t--) {
+                BGK_HIP(launch_adjoint_step<C>(c, lin, lout, tape + (size_t)(t - s0) * 3 * N));
+                std::swap(lin, lout);
+            }
+        }
+    }
+    hipLaunchKernelGGL(k_grad_u0<C>, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, c->s0, lin,
+                       u0, grad, N);
+    BGK_HIP(hipGetLastError());
+    BGK_HIP(hipStreamSynchronize(c->s0));
+    const double bwd_ms = ms_since(t0);
+    if (stats) {
+        stats[0] = (double)a.bytes;
+        stats[1] = fwd_ms;
+        stats[2] = bwd_ms;
+        stats[3] = (double)a.recomputed;
+    }
+    return BGK_OK;
+}
+
+template <typename S, typename C>
+int forward_t(bgk_ctx *c, const double *u0, int T, double *J) {
+    const Lattice &L = c->lat;
+    BGK_HIP(launch_init_equilibrium(L, c->cur, u0, L.nsite, 0, L.nsite, c->s0));
+    for (int t = 0; t < T; t++) {
+        BGK_HIP(launch_collide(L, c->cur, Box{0, L.nx, 0, L.ny, 0, L.nz}, c->omega, c->s0));
+        c->cur ^= 1;
+    }
+    return reduce_ke(c, c->cur, J);
+}
+
+}  // namespace
+
+static int check_adjoint_ctx(bgk_ctx *c, int nsteps) {
+    if (c->lat.layout != BGK_LAYOUT_PERIODIC)
+        return fail(BGK_EINVAL, "adjoint: periodic layout only");
+    if (!c->lat.shifted) return fail(BGK_EINVAL, "adjoint: shifted storage only");
+    if (nsteps < 0) return fail(BGK_EINVAL, "adjoint: nsteps must be >= 0");
+    return BGK_OK;
+}
+
+int adjoint_ke(bgk_ctx *c, const double *u0, int nsteps, int ckpt_every, int levels, double *J,
+               double *grad, double stats[4]) {
+    int r = check_adjoint_ctx(c, nsteps);
+    if (r) return r;
+    if (levels < 0 || levels > 2) return fail(BGK_EINVAL, "adjoint: levels must be 0, 1 or 2");
+    switch (c->lat.storage) {
+    case BGK_STORE_F64: return adjoint_t<double, double>(c, u0, nsteps, ckpt_every, levels, J, grad, stats);
+    case BGK_STORE_F32: return adjoint_t<float, float>(c, u0, nsteps, ckpt_every, levels, J, grad, stats);
+    case BGK_STORE_F32_MIXED:
+        return adjoint_t<float, double>(c, u0, nsteps, ckpt_every, levels, J, grad, stats);
+    default: return fail(BGK_EINVAL, "adjoint: storage must be f64, f32 or f32-mixed");
+    }
+}
+
+int forward_ke(bgk_ctx *c, const double *u0, int nsteps, double *J) {
+    int r = check_adjoint_ctx(c, nsteps);
+    if (r) return r;
+    switch (c->lat.storage) {
+    case BGK_STORE_F64: return forward_t<double, double>(c, u0, nsteps, J);
+    case BGK_STORE_F32: return forward_t<float, float>(c, u0, nsteps, J);
+    case BGK_STORE_F32_MIXED: return forward_t<float, double>(c, u0, nsteps, J);
+    case BGK_STORE_F16: return forward_t<half_t, float>(c, u0, nsteps, J);
+    default: return forward_t<bf16_t, float>(c, u0, nsteps, J);
+    }
 }
 
 }  // namespace bgk

@@ -603,6 +603,19 @@ int bgk_dissipation(bgk_ctx *c, double svisc, int mode, double out[2]) {
     return BGK_OK;
 }
 
+int bgk_checksum(bgk_ctx *c, uint64_t *out) {
+    int r = check_ctx(c);
+    if (r) return r;
+    if (!out) return fail(BGK_EINVAL, "null output");
+    // red_partials holds 8 doubles per block: room for one 64-bit word per block
+    BGK_HIP(launch_checksum(c->lat, c->cur, c->red_partials, c->red_blocks, c->red_out, c->s0));
+    BGK_HIP(hipMemcpyAsync(c->red_host, c->red_out, sizeof(uint64_t), hipMemcpyDeviceToHost,
+                           c->s0));
+    BGK_HIP(hipStreamSynchronize(c->s0));
+    memcpy(out, c->red_host, sizeof(uint64_t));
+    return BGK_OK;
+}
+
 int bgk_synchronize(bgk_ctx *c) {
     int r = check_ctx(c);
     if (r) return r;

@@ -132,6 +132,10 @@ hipError_t launch_plane_to_f64(const Lattice &L, int copy, int q, double *dst, l
 // describing sites i0 .. i0+count-1
 hipError_t launch_init_equilibrium(const Lattice &L, int copy, const double *u, long long ld,
                                    long long i0, long long count, hipStream_t s);
+// order-independent hash of the stored interior populations of copy `cur`
+// (partials: nblocks 64-bit words, out: one 64-bit word, both on the device)
+hipError_t launch_checksum(const Lattice &L, int cur, void *partials, int nblocks, void *out,
+                           hipStream_t s);
 // zero the ghost planes of the slab layout of one copy
 hipError_t launch_fill_zero(void *p, size_t bytes, hipStream_t s);
 

@@ -14,103 +14,9 @@
 //
 // All reductions accumulate in FP64 with a fixed grid and a fixed
 // combination order, so they are deterministic run to run.
-#include "bgk_internal.h"
-#include "d3q19.h"
+#include "bgk_collide.h"
 
 namespace bgk {
-
-template <typename S>
-struct Pops {
-    S *p[19];
-};
-
-template <typename S>
-static Pops<S> pops_of(const Lattice &L, int copy) {
-    Pops<S> P;
-    for (int q = 0; q < 19; q++) P.p[q] = (S *)L.pop[copy][q];
-    return P;
-}
-
-// Launch geometry of a box of sites in 64 x 4 tiles (one z plane deep).
-struct Geo {
-    int nx, ny, nz, nxy, zoff, periodic_z;
-    int tiles_x, tiles_y;
-    int x0, x1, y0, y1, z0;
-};
-
-static Geo make_geo(const Lattice &L, const Box &b, int *ntiles) {
-    Geo g;
-    g.nx = L.nx;
-    g.ny = L.ny;
-    g.nz = L.nz;
-    g.nxy = L.nx * L.ny;
-    g.zoff = L.zoff;
-    g.periodic_z = (L.layout == BGK_LAYOUT_PERIODIC);
-    g.tiles_x = (b.x1 - b.x0 + 63) / 64;
-    g.tiles_y = (b.y1 - b.y0 + 3) / 4;
-    g.x0 = b.x0;
-    g.x1 = b.x1;
-    g.y0 = b.y0;
-    g.y1 = b.y1;
-    g.z0 = b.z0;
-    *ntiles = g.tiles_x * g.tiles_y * (b.z1 - b.z0);
-    return g;
-}
-
-// Workgroups are dispatched round-robin over the 8 XCDs.  Remap so that each
-// XCD works on one contiguous range of tiles: x-adjacent tiles then share an
-// L2 and the cache line that straddles their boundary is fetched once.
-__device__ __forceinline__ int xcd_remap(int b, int G) {
-    const int q = G >> 3, r = G & 7;
-    const int xcd = b & 7, slot = b >> 3;
-    return (xcd < r) ? xcd * (q + 1) + slot : r * (q + 1) + (xcd - r) * q + slot;
-}
-
-// ------------------------------------------------------------------ collide
-template <typename S, typename C>
-__global__ __launch_bounds__(256) void k_collide(Pops<S> src, Pops<S> dst, Geo g, C omega,
-                                                 C cte1) {
-    const int t = xcd_remap((int)blockIdx.x, (int)gridDim.x);
-    const int tx = t % g.tiles_x;
-    const int r = t / g.tiles_x;
-    const int ty = r % g.tiles_y;
-    const int tz = r / g.tiles_y;
-    const int x = g.x0 + tx * 64 + (int)threadIdx.x;
-    const int y = g.y0 + ty * 4 + (int)threadIdx.y;
-    const int z = g.z0 + tz;
-    if (x >= g.x1 || y >= g.y1) return;
-
-    const int xm = (x == 0) ? g.nx - 1 : x - 1;
-    const int xp = (x == g.nx - 1) ? 0 : x + 1;
-    const int ym = (y == 0) ? g.ny - 1 : y - 1;
-    const int yp = (y == g.ny - 1) ? 0 : y + 1;
-    const int zs = z + g.zoff;
-    int zm, zp;
-    if (g.periodic_z) {
-        zm = (z == 0) ? g.nz - 1 : z - 1;
-        zp = (z == g.nz - 1) ? 0 : z + 1;
-    } else {  // slab: ghost planes 0 and nz+1 hold the neighbours' images
-        zm = zs - 1;
-        zp = zs + 1;
-    }
-    const int X[3] = {xm, x, xp};
-    const int Y[3] = {ym * g.nx, y * g.nx, yp * g.nx};
-    const int Z[3] = {zm * g.nxy, zs * g.nxy, zp * g.nxy};
-
-    C f[19];
-#pragma unroll
-    for (int q = 0; q < 19; q++) {
-        // pull: population q arrives from x - c_q
-        const int idx = X[1 - cx(q)] + Y[1 - cy(q)] + Z[1 - cz(q)];
-        f[q] = Storage<S>::template load<C>(src.p[q][idx]);
-    }
-    C b[19];
-    collide_site<C>(f, b, omega, cte1);
-    const int c = x + Y[1] + Z[1];
-#pragma unroll
-    for (int q = 0; q < 18; q++) dst.p[q][c] = Storage<S>::template store<C>(b[q]);
-    src.p[18][c] = Storage<S>::template store<C>(b[18]);  // rest: in place
-}
 
 template <typename S, typename C>
 static hipError_t collide_t(const Lattice &L, int src, const Box &box, double omega,
@@ -270,6 +176,73 @@ static hipError_t dissipation_t(const Lattice &L, int cur, const void *gcache, d
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_finalize, dim3(1), dim3(RED_THREADS), 0, s, partials, nblocks, out);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------- checksum
+// Order-independent 64-bit hash of the raw stored bits of the interior
+// populations: sum over (q, site) of splitmix64(bits ^ key(q, site)) mod 2^64.
+__device__ __forceinline__ unsigned long long mix64(unsigned long long x) {
+    x += 0x9E3779B97F4A7C15ull;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+    return x ^ (x >> 31);
+}
+template <typename S>
+__device__ __forceinline__ unsigned long long raw_bits(S v) {
+    if constexpr (sizeof(S) == 8) return __builtin_bit_cast(unsigned long long, v);
+    else if constexpr (sizeof(S) == 4) return __builtin_bit_cast(unsigned int, v);
+    else return __builtin_bit_cast(unsigned short, v);
+}
+
+template <typename S>
+__global__ __launch_bounds__(RED_THREADS) void k_checksum(Pops<S> P, long long nsite,
+                                                          long long off,
+                                                          unsigned long long *partials) {
+    unsigned long long h = 0;
+    for (long long i = (long long)blockIdx.x * RED_THREADS + threadIdx.x; i < nsite;
+         i += (long long)gridDim.x * RED_THREADS) {
+#pragma unroll
+        for (int q = 0; q < 19; q++)
+            h += mix64(raw_bits(P.p[q][off + i]) ^ ((unsigned long long)q << 58) ^
+                       ((unsigned long long)i * 0xD1B54A32D192ED03ull));
+    }
+    for (int o = 32; o > 0; o >>= 1) h += __shfl_xor(h, o);
+    __shared__ unsigned long long sm[RED_THREADS / 64];
+    if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = h;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long t = 0;
+        for (int w = 0; w < RED_THREADS / 64; w++) t += sm[w];
+        partials[blockIdx.x] = t;
+    }
+}
+
+__global__ __launch_bounds__(RED_THREADS) void k_checksum_final(const unsigned long long *partials,
+                                                                int nblocks,
+                                                                unsigned long long *out) {
+    unsigned long long h = 0;
+    for (int b = threadIdx.x; b < nblocks; b += RED_THREADS) h += partials[b];
+    for (int o = 32; o > 0; o >>= 1) h += __shfl_xor(h, o);
+    __shared__ unsigned long long sm[RED_THREADS / 64];
+    if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = h;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long t = 0;
+        for (int w = 0; w < RED_THREADS / 64; w++) t += sm[w];
+        *out = t;
+    }
+}
+
+template <typename S>
+static hipError_t checksum_t(const Lattice &L, int cur, void *partials, int nblocks, void *out,
+                             hipStream_t s) {
+    hipLaunchKernelGGL((k_checksum<S>), dim3(nblocks), dim3(RED_THREADS), 0, s, pops_of<S>(L, cur),
+                       L.nsite, (long long)L.zoff * L.nx * L.ny, (unsigned long long *)partials);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_checksum_final, dim3(1), dim3(RED_THREADS), 0, s,
+                       (const unsigned long long *)partials, nblocks, (unsigned long long *)out);
     return hipGetLastError();
 }
 
@@ -497,6 +470,10 @@ hipError_t launch_plane_to_f64(const Lattice &L, int copy, int q, double *dst, l
 hipError_t launch_init_equilibrium(const Lattice &L, int copy, const double *u, long long ld,
                                    long long i0, long long count, hipStream_t s) {
     BGK_DISPATCH(L.storage, init_eq_t, L, copy, u, ld, i0, count, s);
+}
+hipError_t launch_checksum(const Lattice &L, int cur, void *partials, int nblocks, void *out,
+                           hipStream_t s) {
+    BGK_DISPATCH_S(L.storage, checksum_t, L, cur, partials, nblocks, out, s);
 }
 hipError_t launch_fill_zero(void *p, size_t bytes, hipStream_t s) {
     return hipMemsetAsync(p, 0, bytes, s);

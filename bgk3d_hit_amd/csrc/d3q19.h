@@ -115,8 +115,11 @@ struct K {
 // populations, b[] receives the post-collision populations (b[18] is the
 // rest population the reference updates in place).  cte1 = 1 for shifted
 // storage (SRC/col_MC.F90:86-91).
+// ux, uy, uz receive the pulled-moment velocity j/rho (before the
+// reference scales it by rf for the linear terms).
 template <typename C>
-__device__ __forceinline__ void collide_site(const C (&x)[19], C (&b)[19], C omega, C cte1) {
+__device__ __forceinline__ void collide_site(const C (&x)[19], C (&b)[19], C omega, C cte1, C &ux,
+                                             C &uy, C &uz) {
     using k = K<C>;
     const C cte0 = k::uno - cte1;
     const C x01 = x[0], x02 = x[1], x03 = x[2], x04 = x[3], x05 = x[4], x06 = x[5], x07 = x[6];
@@ -130,6 +133,9 @@ __device__ __forceinline__ void collide_site(const C (&x)[19], C (&b)[19], C ome
     C vy = (x03 + x07 + x08 + x09 + x12 - x01 - x10 - x16 - x17 - x18) * rhoinv;
     C vz = (x04 + x06 + x07 + x13 + x18 - x02 - x09 - x11 - x15 - x16) * rhoinv;
 
+    ux = vx;
+    uy = vy;
+    uz = vz;
     const C vx2 = vx * vx, vy2 = vy * vy, vz2 = vz * vz;
     const C vsq = vx2 + vy2 + vz2;
     C vxpy = vx + vy;

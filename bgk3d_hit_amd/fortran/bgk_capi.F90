@@ -137,6 +137,12 @@ module bgk_capi
        real(c_double), intent(out) :: out(2)
      end function bgk_dissipation
 
+     integer(c_int) function bgk_checksum(ctx, out) bind(c, name='bgk_checksum')
+       import :: c_int, c_ptr, c_int64_t
+       type(c_ptr), value :: ctx
+       integer(c_int64_t), intent(out) :: out
+     end function bgk_checksum
+
      integer(c_int) function bgk_synchronize(ctx) bind(c, name='bgk_synchronize')
        import :: c_int, c_ptr
        type(c_ptr), value :: ctx

@@ -158,6 +158,12 @@ class Lattice:
                                           else L.DIAG_CONSISTENT, _dptr(out)))
         return out
 
+    def checksum(self):
+        """64-bit order-independent hash of the stored populations."""
+        v = ctypes.c_uint64(0)
+        L.check(self._lib.bgk_checksum(self._h, ctypes.byref(v)))
+        return v.value
+
     def synchronize(self):
         L.check(self._lib.bgk_synchronize(self._h))
 
@@ -179,6 +185,18 @@ class Lattice:
                                          int(levels), ctypes.byref(J), _dptr(g), _dptr(st)))
         return J.value, g, {"peak_bytes": st[0], "forward_ms": st[1], "backward_ms": st[2],
                             "recomputed_steps": int(st[3])}
+
+    def adjoint_ke_device(self, u0_ptr, grad_ptr, nsteps, ckpt_every=25, levels=1):
+        """Device-pointer variant: u0_ptr / grad_ptr address 3*nsite doubles in
+        GPU memory (e.g. torch tensors' data_ptr()).  The call returns after
+        the gradient is complete."""
+        J = ctypes.c_double(0.0)
+        st = np.zeros(4)
+        L.check(self._lib.bgk_adjoint_ke_device(self._h, ctypes.c_void_p(int(u0_ptr)), int(nsteps),
+                                                int(ckpt_every), int(levels), ctypes.byref(J),
+                                                ctypes.c_void_p(int(grad_ptr)), _dptr(st)))
+        return J.value, {"peak_bytes": st[0], "forward_ms": st[1], "backward_ms": st[2],
+                         "recomputed_steps": int(st[3])}
 
     def forward_ke(self, u0, nsteps):
         u0 = self._in(u0, 3)

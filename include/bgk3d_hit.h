@@ -147,6 +147,14 @@ int bgk_diagno(bgk_ctx *ctx, int mode, double out[5]);
  * reference's loctot(1..2), before its output normalisation).              */
 int bgk_dissipation(bgk_ctx *ctx, double svisc, int mode, double out[2]);
 
+/* Order-independent 64-bit hash of the raw stored bits of the current
+ * interior populations (equal hashes <=> bit-identical states, up to hash
+ * collisions).  No reference counterpart: the reference compares states
+ * through its probe dumps (SRC/probe_global.F90:55-95); this lets a caller
+ * check bit identity of two lattices (slab vs periodic, restart) without
+ * downloading them.                                                         */
+int bgk_checksum(bgk_ctx *ctx, uint64_t *out);
+
 /* wait for all device work of the context */
 int bgk_synchronize(bgk_ctx *ctx);
 

@@ -1,0 +1,87 @@
+"""Discrete adjoint on the GPU: against the oracle's FP64 adjoint (a
+generic-table restatement, itself pinned by finite differences on the CPU),
+against central finite differences of the GPU forward, across checkpoint
+levels, and through the torch.autograd binding."""
+import numpy as np
+import pytest
+
+import bgk3d_hit_amd as bgk
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print("[gpu-case]", *a, flush=True)
+
+
+@pytest.mark.parametrize("levels,K", [(0, 1), (1, 5), (2, 3)])
+def test_adjoint_f64_matches_oracle(gpu, levels, K):
+    shape, omega, T = (12, 10, 8), 1.8, 13
+    say("adjoint vs oracle", levels, K)
+    u0 = bgk.hit_velocity(*shape, 2.0, 0.03, 21)
+    J_o, g_o = oracle.adjoint_ke(*shape, omega, T, u0)
+    lat = bgk.Lattice(*shape, omega=omega, storage="f64")
+    J, g, st = lat.adjoint_ke(u0, T, ckpt_every=K, levels=levels)
+    assert abs(J - J_o) < 1e-13 * J_o
+    assert np.abs(g - g_o).max() < 1e-10 * np.abs(g_o).max()
+    if levels == 0:
+        assert st["recomputed_steps"] == 0
+
+
+def test_adjoint_checkpoint_levels_agree_bitwise(gpu):
+    shape, omega, T = (24, 24, 24), 1.9, 30
+    say("adjoint levels bitwise")
+    u0 = bgk.hit_velocity(*shape, 3.0, 0.02, 5)
+    lat = bgk.Lattice(*shape, omega=omega, storage="f32")
+    res = [lat.adjoint_ke(u0, T, ckpt_every=K, levels=lv) for lv, K in ((0, 1), (1, 4), (2, 4))]
+    for J, g, st in res[1:]:
+        assert J == res[0][0]
+        assert np.array_equal(g, res[0][1])
+    # one level: every segment but the last (taped by the forward) is re-run
+    assert res[1][2]["recomputed_steps"] == ((T + 3) // 4 - 1) * 4
+    # two levels: 3 coarse segments of 12 steps; fine checkpoints re-run too
+    assert res[2][2]["recomputed_steps"] == 44
+
+
+def test_adjoint_central_differences_fp64(gpu):
+    shape, omega, T = (32, 32, 32), 1.9, 100
+    say("adjoint FD fp64")
+    u0 = bgk.hit_velocity(*shape, 4.0, 0.02, 42)
+    lat = bgk.Lattice(*shape, omega=omega, storage="f64")
+    J, g, _ = lat.adjoint_ke(u0, T, ckpt_every=25, levels=1)
+    assert abs(lat.forward_ke(u0, T) - J) < 1e-13 * J
+    eps = 1e-3
+    for s in range(1, 5):
+        d = bgk.hit_velocity(*shape, 4.0, 0.02, 1000 + s)
+        fd = (lat.forward_ke(u0 + eps * d, T) - lat.forward_ke(u0 - eps * d, T)) / (2 * eps)
+        ad = float((g * d).sum())
+        assert abs(fd - ad) < 1e-6 * abs(fd), (s, fd, ad)
+
+
+def test_adjoint_fp32_against_fp64_cpu_adjoint(gpu):
+    """FP32 GPU adjoint against the FP64 CPU adjoint at 32^3."""
+    shape, omega, T = (32, 32, 32), 1.90, 60
+    say("adjoint fp32 vs cpu fp64")
+    u0 = bgk.hit_velocity(*shape, 4.0, 0.02, 42)
+    J_o, g_o = oracle.adjoint_ke(*shape, omega, T, u0)
+    lat = bgk.Lattice(*shape, omega=omega, storage="f32")
+    J, g, _ = lat.adjoint_ke(u0, T, ckpt_every=25, levels=1)
+    assert abs(J - J_o) < 1e-5 * J_o
+    err = np.sqrt(((g - g_o) ** 2).sum() / (g_o ** 2).sum())
+    assert err < 1e-4, err
+
+
+def test_torch_autograd_binding(gpu):
+    torch = pytest.importorskip("torch")
+    from bgk3d_hit_amd.torch_autograd import kinetic_energy
+    shape, omega, T = (16, 16, 16), 1.9, 20
+    say("torch autograd")
+    u0n = bgk.hit_velocity(*shape, 2.0, 0.02, 3)
+    lat = bgk.Lattice(*shape, omega=omega, storage="f64")
+    J_ref, g_ref, _ = lat.adjoint_ke(u0n, T, ckpt_every=5, levels=1)
+    u0 = torch.tensor(u0n, device="cuda", requires_grad=True)
+    J = kinetic_energy(u0, lat, T, ckpt_every=5, levels=1)
+    (2.0 * J).backward()
+    assert abs(J.item() - J_ref) < 1e-14 * J_ref
+    assert np.array_equal(u0.grad.cpu().numpy(), 2.0 * g_ref)

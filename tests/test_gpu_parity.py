@@ -244,3 +244,55 @@ def test_512_cubed_fp32_against_fp64(gpu):
     assert np.abs(e1[1:4] - d1[1:4]).max() < 1e-5 * 0.02
     assert abs(e1[4] - d1[4]) < 1e-5 * d1[4]
     assert np.abs(s32 - s64).max() < 1e-4 * np.abs(s64).max()
+
+
+def test_checksum_sees_one_ulp(gpu):
+    g = bgk.Lattice(32, 32, 32, omega=1.9, storage="f32")
+    g.init_hit(4.0, 0.02, 7)
+    c0 = g.checksum()
+    assert c0 == g.checksum()
+    f = g.get_populations()
+    v = np.float32(f[3, 4, 5, 6])
+    f2 = f.copy()
+    f2[3, 4, 5, 6] = np.nextafter(v, np.float32(1))
+    g.set_populations(f2)
+    assert g.checksum() != c0
+    g.set_populations(f)
+    assert g.checksum() == c0
+
+
+def test_config2_128cubed_fp64_against_oracle(gpu):
+    """128^3 HIT, omega=1.95, FP64: GPU against the CPU oracle, diagnostics
+    every 50 steps."""
+    shape, omega = (128, 128, 128), 1.95
+    nu = bgk.svisc_from_omega(omega)
+    say("128^3 FP64 vs oracle")
+    g, o, _ = hit_pair(shape, 4.0, 0.02, 1234, omega, "f64")
+    for it in range(50, 201, 50):
+        g.step(50)
+        o.step(50)
+        dg, do = g.diagno(), o.diagno()
+        assert abs(dg[0] - do[0]) < 1e-12
+        assert abs(dg[4] - do[4]) < 1e-10 * do[4]
+        o.bcond_step3()
+        sg, so = g.dissipation(nu), o.dissipation(nu)
+        assert np.abs(sg - so).max() < 1e-10 * np.abs(so).max()
+    assert np.array_equal(g.get_populations(), o.get_populations())
+
+
+def test_config3_slab_1024x1024x256_bitwise(gpu):
+    """One rank's share at its real shape: 1024x1024x256 FP32 slab with
+    +-z ghost planes and the two-stream overlap, bit-identical to the
+    periodic kernel (k0=8, u_rms=0.03, omega=1.97)."""
+    shape = (1024, 1024, 256)
+    say("slab 1024x1024x256")
+    p = bgk.Lattice(*shape, omega=1.97, storage="f32")
+    p.init_hit(8.0, 0.03, 5678)
+    p.step(10)
+    cp, dp_ = p.checksum(), p.diagno()
+    p.close()
+    s = bgk.Lattice(*shape, omega=1.97, storage="f32", layout="slab")
+    s.init_hit(8.0, 0.03, 5678)
+    s.step(10)
+    assert s.checksum() == cp
+    assert np.array_equal(s.diagno(), dp_)
