@@ -7,10 +7,22 @@
 adjoint pass also produces J, so forward() runs it once and backward() only
 scales the stored gradient.  Torch is plumbing here: the computation is the
 HIP library's.
-"""
-import torch
 
-from .lattice import Lattice
+HIP runtime sharing: the torch wheel carries its own libamdhip64.so.7.
+Import this module (or torch) before the first call into libbgk3d_hit.so so
+that the library binds to the runtime torch already loaded; the reverse
+order leaves two HIP runtimes in the process and torch finds no GPU.
+"""
+import sys
+
+if "bgk3d_hit_amd._lib" in sys.modules and sys.modules["bgk3d_hit_amd._lib"]._lib is not None \
+        and "torch" not in sys.modules:
+    raise ImportError("import torch (or bgk3d_hit_amd.torch_autograd) before the first use of "
+                      "libbgk3d_hit.so: the library must bind to torch's HIP runtime")
+
+import torch  # noqa: E402
+
+from .lattice import Lattice  # noqa: E402
 
 
 class KineticEnergy(torch.autograd.Function):
