@@ -38,13 +38,15 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def run_gpu(kind, n, steps, warmup, diag_every, omega=1.95):
+def run_gpu(kind, n, steps, warmup, diag_every, omega=1.95, shape=None, layout="periodic",
+            k0=4.0, urms=0.02, seed=1234):
     import bgk3d_hit_amd as bgk
 
+    shape = shape or (n, n, n)
     nu = bgk.svisc_from_omega(omega)
-    lat = bgk.Lattice(n, n, n, omega=omega, storage=kind, timing=True)
+    lat = bgk.Lattice(*shape, omega=omega, storage=kind, layout=layout, timing=True)
     try:
-        lat.init_hit(4.0, 0.02, 1234)
+        lat.init_hit(k0, urms, seed)
         d0 = lat.diagno()
         lat.step(warmup)
         lat.synchronize()
@@ -63,16 +65,23 @@ def run_gpu(kind, n, steps, warmup, diag_every, omega=1.95):
         t = lat.timing(reset=True)
     finally:
         lat.close()
-    nsite = n ** 3
-    avg_ms = t["collide_ms"] / max(t["collide_launches"], 1)
+    nsite = shape[0] * shape[1] * shape[2]
+    launches = max(t["collide_launches"], 1)
+    # per step: one collide launch (periodic) or bulk + 2 boundary planes (slab)
+    per_step = launches / steps
+    avg_ms = t["collide_ms"] / launches
+    step_collide_ms = t["collide_ms"] / steps
     algo_bytes = 38 * BYTES[kind] * nsite
-    achieved = algo_bytes / (avg_ms * 1e-3) / 1e9
+    achieved = algo_bytes / (step_collide_ms * 1e-3) / 1e9
     return {
         "kind": kind,
         "mlups": nsite * steps / wall / 1e6,
         "ms_per_step": wall * 1e3 / steps,
         "collide_avg_ms": avg_ms,
+        "collide_ms_per_step": step_collide_ms,
+        "collide_launches_per_step": per_step,
         "collide_launches": t["collide_launches"],
+        "halo_ms_per_step": t["halo_ms"] / steps,
         "reduce_ms_total": t["reduce_ms"],
         "achieved_gbs": achieved,
         "frac": achieved / HBM_PEAK_GBS,
@@ -82,7 +91,62 @@ def run_gpu(kind, n, steps, warmup, diag_every, omega=1.95):
     }
 
 
-def cpu_baseline(min_seconds=10.0, max_steps=8):
+def summary(r, ref=None):
+    out = {"mlups": round(r["mlups"], 1), "ms_per_step": round(r["ms_per_step"], 4),
+           "collide_ms_per_step": round(r["collide_ms_per_step"], 4),
+           "achieved_gbs": round(r["achieved_gbs"], 1), "frac": round(r["frac"], 4),
+           "storage_bytes": BYTES[r["kind"]], "compute": COMPUTE[r["kind"]],
+           "traffic": load_traffic(r["kind"])}
+    if ref is not None:
+        out["speedup_vs_f32"] = round(r["mlups"] / ref["mlups"], 3)
+    return out
+
+
+def guarded(name, fn):
+    try:
+        return fn()
+    except Exception as e:  # an extra configuration must not cost the headline line
+        log(f"[bench] {name} failed: {e}")
+        return {"error": str(e)[:300]}
+
+
+def slab_config(steps, warmup):
+    r = run_gpu("f32", 0, steps, warmup, 50, omega=1.97, shape=(1024, 1024, 256), layout="slab",
+                k0=8.0, urms=0.03, seed=5678)
+    s = summary(r)
+    s.update({"workload": "one rank's slab 1024x1024x256 FP32, +-z ghost planes, pack / "
+                          "self-exchange / unpack, bulk || boundary on two HIP streams, "
+                          "k0=8 u_rms=0.03 omega=1.97",
+              "halo_ms_per_step": round(r["halo_ms_per_step"], 4),
+              "collide_launches_per_step": r["collide_launches_per_step"]})
+    return s
+
+
+def c4_config(steps, warmup):
+    res = {}
+    ref = None
+    for kind in ("f32", "f16", "bf16"):
+        r = run_gpu(kind, 1024, steps, warmup, 50, omega=1.97, k0=8.0, urms=0.03, seed=5678)
+        if kind == "f32":
+            ref = r
+        res[kind] = summary(r, ref)
+        log(f"[bench] 1024^3 {kind}: {r['mlups']:.0f} MLUPS")
+    res["workload"] = "1024^3 periodic HIT k0=8 u_rms=0.03 omega=1.97, shifted storage"
+    return res
+
+
+def c5_config():
+    from bgk3d_hit_amd.studies import adjoint_study
+    r = adjoint_study(n=128, steps=500, ckpt=25, levels=1, fd=False)
+    return {"workload": "128^3 FP32 HIT k0=4 u_rms=0.02 omega=1.90, 500-step horizon, "
+                        "checkpoints every 25 steps with recompute",
+            "forward_ms": round(r["forward_ms"], 2),
+            "adjoint_total_ms": round(r["adjoint_total_ms"], 2),
+            "ratio_adjoint_to_forward": round(r["ratio_adjoint_to_forward"], 3),
+            "peak_hbm_bytes": r["peak_hbm_bytes"]}
+
+
+def cpu_baseline(min_seconds=10.0, max_steps=400):
     import numpy as np
 
     import bgk3d_hit_amd as bgk
@@ -148,6 +212,8 @@ def main():
     ap.add_argument("--variants", default="f64,f16,bf16",
                     help="extra storage kinds to measure beside f32 ('' for none)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="skip the slab, 1024^3 and adjoint configurations")
     ap.add_argument("--calibrate-copy", type=float, default=0.0, metavar="GIB",
                     help="only time a plain device copy of GIB GiB (the reference kernel "
                          "for scaling rocprofv3 FETCH_SIZE/WRITE_SIZE) and exit")
@@ -171,20 +237,25 @@ def main():
     variants = {}
     for kind in [k for k in a.variants.split(",") if k]:
         log(f"[bench] {kind} {a.size}^3")
-        r = run_gpu(kind, a.size, a.steps, a.warmup, a.diag_every)
+        r = guarded(kind, lambda: run_gpu(kind, a.size, a.steps, a.warmup, a.diag_every))
+        if "error" in r:
+            variants[kind] = r
+            continue
         log(f"[bench] {kind}: {r['mlups']:.0f} MLUPS, {r['achieved_gbs']:.0f} GB/s")
-        variants[kind] = {
-            "mlups": round(r["mlups"], 1), "ms_per_step": round(r["ms_per_step"], 4),
-            "collide_avg_ms": round(r["collide_avg_ms"], 4),
-            "achieved_gbs": round(r["achieved_gbs"], 1), "frac": round(r["frac"], 4),
-            "storage_bytes": BYTES[kind], "compute": COMPUTE[kind],
-            "traffic": load_traffic(kind),
-            "speedup_vs_f32": round(r["mlups"] / main_run["mlups"], 3)}
+        variants[kind] = summary(r, main_run)
+    configs = {}
+    if not a.no_extra:
+        log("[bench] slab 1024x1024x256")
+        configs["slab_1024x1024x256"] = guarded("slab", lambda: slab_config(100, 10))
+        log("[bench] 1024^3 f32/f16/bf16")
+        configs["periodic_1024cubed"] = guarded("1024^3", lambda: c4_config(100, 10))
+        log("[bench] adjoint 128^3")
+        configs["adjoint_128cubed"] = guarded("adjoint", c5_config)
     cpu = None
     if not a.no_cpu:
         log("[bench] cpu baseline")
-        cpu = cpu_baseline()
-        log(f"[bench] cpu: {cpu['value']} MLUPS on {cpu['cores']} threads")
+        cpu = guarded("cpu", cpu_baseline)
+        log(f"[bench] cpu: {cpu}")
     out = {
         "metric": METRIC,
         "value": round(main_run["mlups"], 1),
@@ -211,6 +282,7 @@ def main():
                      "algo_bytes_per_launch": main_run["algo_bytes_per_launch"]},
         "cpu_baseline": cpu,
         "variants": variants,
+        "configs": configs,
     }
     print(json.dumps(out), flush=True)
 

@@ -8,11 +8,13 @@
 //   rho = sum g + 1, j = sum c g, u = j / rho.
 // Adjoint step: lambda^t_k(x - c_k) = mu_k(x) with
 //   mu_k = (1 - omega) lambda_k + omega [A + (B - 3 S u) . c_k],
+// stored as M^t_k(x) = mu_k(x), so that lambda^t_i(x) = M^t_i(x + c_i): each
+// step reads M^{t+1} with shifted loads and writes M^t at the site (the
+// forward's own access pattern: aligned stores, no push),
 //   A = sum_i w_i lambda_i (1 - 9/2 (c_i.u)^2 + 3/2 |u|^2),
 //   B = sum_i w_i lambda_i (3 + 9 c_i.u) c_i,   S = sum_i w_i lambda_i,
 // i.e. J^T applied per site as a rank-4 moment correction plus the
-// (1 - omega) diagonal, then pushed back along the streaming permutation
-// (every target written exactly once: no atomics).  The Jacobian depends on
+// (1 - omega) diagonal.  The Jacobian depends on
 // the site's pulled-moment velocity u only, so the forward pass records u
 // (3 values per site and step, the "tape") instead of whole states.
 // Final condition: lambda^T_i = (u.c_i - |u|^2) / rho (at-site moments).
@@ -34,11 +36,14 @@
 
 namespace bgk {
 
-// lambda^T from at-site moments of the final state
+// lambda^T from at-site moments of the final state, stored in the shifted
+// form M^T_i(x + c_i) = lambda^T_i(x)
 template <typename S, typename C>
-__global__ __launch_bounds__(256) void k_lambda_final(Pops<S> P, C *lam, long long nsite) {
+__global__ __launch_bounds__(256) void k_lambda_final(Pops<S> P, C *M, int nx, int ny, int nz) {
+    const long long nsite = (long long)nx * ny * nz;
     const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
     if (i >= nsite) return;
+    const int x = (int)(i % nx), y = (int)((i / nx) % ny), z = (int)(i / ((long long)nx * ny));
     C f[19];
 #pragma unroll
     for (int q = 0; q < 19; q++) f[q] = Storage<S>::template load<C>(P.p[q][i]);
@@ -46,18 +51,24 @@ __global__ __launch_bounds__(256) void k_lambda_final(Pops<S> P, C *lam, long lo
     moments_site<C>(f, (C)1, rho, ux, uy, uz);
     const C usq = ux * ux + uy * uy + uz * uz;
 #pragma unroll
-    for (int q = 0; q < 19; q++)
-        lam[(size_t)q * nsite + i] = (ux * (C)cx(q) + uy * (C)cy(q) + uz * (C)cz(q) - usq) / rho;
+    for (int q = 0; q < 19; q++) {
+        int tx = x + cx(q), ty = y + cy(q), tz = z + cz(q);
+        tx = tx < 0 ? tx + nx : (tx >= nx ? tx - nx : tx);
+        ty = ty < 0 ? ty + ny : (ty >= ny ? ty - ny : ty);
+        tz = tz < 0 ? tz + nz : (tz >= nz ? tz - nz : tz);
+        M[(size_t)q * nsite + tx + (size_t)nx * (ty + (size_t)ny * tz)] =
+            (ux * (C)cx(q) + uy * (C)cy(q) + uz * (C)cz(q) - usq) / rho;
+    }
 }
 
-// one adjoint step: reads lambda^{t+1} at site x and the tape u^t(x),
-// writes lambda^t_k at x - c_k
+// one adjoint step: lambda^{t+1}_i(x) = M^{t+1}_i(x + c_i) (shifted loads),
+// tape u^t(x); writes M^t_k(x) = mu_k(x)
 template <typename C>
-__global__ __launch_bounds__(256) void k_adjoint_step(const C *__restrict__ lin,
-                                                      C *__restrict__ lout,
+__global__ __launch_bounds__(256) void k_adjoint_step(const C *__restrict__ Min,
+                                                      C *__restrict__ Mout,
                                                       const C *__restrict__ tape, int nx, int ny,
                                                       int nz, int tiles_x, int tiles_y, C omega) {
-    const int t = (int)blockIdx.x;
+    const int t = xcd_remap((int)blockIdx.x, (int)gridDim.x);
     const int tx = t % tiles_x;
     const int r = t / tiles_x;
     const int ty = r % tiles_y;
@@ -67,12 +78,19 @@ __global__ __launch_bounds__(256) void k_adjoint_step(const C *__restrict__ lin,
     if (x >= nx || y >= ny) return;
     const long long N = (long long)nx * ny * nz;
     const int nxy = nx * ny;
-    const int c = x + nx * y + nxy * z;
-    const C ux = tape[c], uy = tape[N + c], uz = tape[2 * N + c];
-    const C usq = ux * ux + uy * uy + uz * uz;
+    const int xm = (x == 0) ? nx - 1 : x - 1, xp = (x == nx - 1) ? 0 : x + 1;
+    const int ym = (y == 0) ? ny - 1 : y - 1, yp = (y == ny - 1) ? 0 : y + 1;
+    const int zm = (z == 0) ? nz - 1 : z - 1, zp = (z == nz - 1) ? 0 : z + 1;
+    const int X[3] = {xm, x, xp};
+    const int Y[3] = {ym * nx, y * nx, yp * nx};
+    const int Z[3] = {zm * nxy, z * nxy, zp * nxy};
     C lam[19];
 #pragma unroll
-    for (int q = 0; q < 19; q++) lam[q] = lin[(size_t)q * N + c];
+    for (int q = 0; q < 19; q++)
+        lam[q] = Min[(size_t)q * N + X[1 + cx(q)] + Y[1 + cy(q)] + Z[1 + cz(q)]];
+    const int c = x + Y[1] + Z[1];
+    const C ux = tape[c], uy = tape[N + c], uz = tape[2 * N + c];
+    const C usq = ux * ux + uy * uy + uz * uz;
     C A = 0, Bx = 0, By = 0, Bz = 0, Sw = 0;
 #pragma unroll
     for (int q = 0; q < 19; q++) {
@@ -86,32 +104,31 @@ __global__ __launch_bounds__(256) void k_adjoint_step(const C *__restrict__ lin,
         Sw += wl;
     }
     const C Gx = Bx - (C)3 * Sw * ux, Gy = By - (C)3 * Sw * uy, Gz = Bz - (C)3 * Sw * uz;
-    const int xm = (x == 0) ? nx - 1 : x - 1, xp = (x == nx - 1) ? 0 : x + 1;
-    const int ym = (y == 0) ? ny - 1 : y - 1, yp = (y == ny - 1) ? 0 : y + 1;
-    const int zm = (z == 0) ? nz - 1 : z - 1, zp = (z == nz - 1) ? 0 : z + 1;
-    const int X[3] = {xm, x, xp};
-    const int Y[3] = {ym * nx, y * nx, yp * nx};
-    const int Z[3] = {zm * nxy, z * nxy, zp * nxy};
 #pragma unroll
-    for (int q = 0; q < 19; q++) {
-        const C mu = ((C)1 - omega) * lam[q] +
-                     omega * (A + Gx * (C)cx(q) + Gy * (C)cy(q) + Gz * (C)cz(q));
-        // population q at x came from x - c_q
-        lout[(size_t)q * N + X[1 - cx(q)] + Y[1 - cy(q)] + Z[1 - cz(q)]] = mu;
-    }
+    for (int q = 0; q < 19; q++)
+        Mout[(size_t)q * N + c] = ((C)1 - omega) * lam[q] +
+                                  omega * (A + Gx * (C)cx(q) + Gy * (C)cy(q) + Gz * (C)cz(q));
 }
 
-// dJ/du0 = sum_i lambda^0_i w_i (3 c_i + 9 (c_i.u0) c_i - 3 u0)
+// dJ/du0 = sum_i lambda^0_i w_i (3 c_i + 9 (c_i.u0) c_i - 3 u0),
+// lambda^0_i(x) = M^0_i(x + c_i)
 template <typename C>
-__global__ __launch_bounds__(256) void k_grad_u0(const C *lam, const double *u0, double *grad,
-                                                 long long N) {
+__global__ __launch_bounds__(256) void k_grad_u0(const C *M, const double *u0, double *grad,
+                                                 int nx, int ny, int nz) {
+    const long long N = (long long)nx * ny * nz;
     const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
     if (i >= N) return;
+    const int x = (int)(i % nx), y = (int)((i / nx) % ny), z = (int)(i / ((long long)nx * ny));
     const double ux = u0[i], uy = u0[N + i], uz = u0[2 * N + i];
     double gx = 0, gy = 0, gz = 0;
 #pragma unroll
     for (int q = 0; q < 19; q++) {
-        const double L = (double)lam[(size_t)q * N + i] * wq(q);
+        int sx = x + cx(q), sy = y + cy(q), sz = z + cz(q);
+        sx = sx < 0 ? sx + nx : (sx >= nx ? sx - nx : sx);
+        sy = sy < 0 ? sy + ny : (sy >= ny ? sy - ny : sy);
+        sz = sz < 0 ? sz + nz : (sz >= nz ? sz - nz : sz);
+        const double L =
+            (double)M[(size_t)q * N + sx + (size_t)nx * (sy + (size_t)ny * sz)] * wq(q);
         const double cu = cx(q) * ux + cy(q) * uy + cz(q) * uz;
         gx += L * (3.0 * cx(q) + 9.0 * cu * cx(q) - 3.0 * ux);
         gy += L * (3.0 * cy(q) + 9.0 * cu * cy(q) - 3.0 * uy);
@@ -234,7 +251,7 @@ int adjoint_t(bgk_ctx *c, const double *u0, int T, int K, int levels, double *J,
     }
     if ((r = reduce_ke(c, c->cur, J))) return r;  // synchronises
     hipLaunchKernelGGL((k_lambda_final<S, C>), dim3((unsigned)((N + 255) / 256)), dim3(256), 0,
-                       c->s0, pops_of<S>(L, c->cur), lamA, N);
+                       c->s0, pops_of<S>(L, c->cur), lamA, L.nx, L.ny, L.nz);
     BGK_HIP(hipGetLastError());
     BGK_HIP(hipStreamSynchronize(c->s0));
     const double fwd_ms = ms_since(t0);
@@ -285,7 +302,7 @@ int adjoint_t(bgk_ctx *c, const double *u0, int T, int K, int levels, double *J,
         }
     }
     hipLaunchKernelGGL(k_grad_u0<C>, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, c->s0, lin,
-                       u0, grad, N);
+                       u0, grad, L.nx, L.ny, L.nz);
     BGK_HIP(hipGetLastError());
     BGK_HIP(hipStreamSynchronize(c->s0));
     const double bwd_ms = ms_since(t0);
