@@ -603,6 +603,39 @@ int bgk_dissipation(bgk_ctx *c, double svisc, int mode, double out[2]) {
     return BGK_OK;
 }
 
+int bgk_get_box(bgk_ctx *c, int x0, int x1, int y0, int y1, int z0, int z1, double *out) {
+    int r = check_ctx(c);
+    if (r) return r;
+    const Lattice &L = c->lat;
+    if (!out) return fail(BGK_EINVAL, "null output");
+    if (x0 < 0 || y0 < 0 || z0 < 0 || x1 > L.nx || y1 > L.ny || z1 > L.nz || x0 >= x1 ||
+        y0 >= y1 || z0 >= z1)
+        return fail(BGK_EINVAL, "box outside the lattice or empty");
+    const Box b{x0, x1, y0, y1, z0, z1};
+    const long long n = b.sites();
+    const long long chunk = std::min(L.nsite, CHUNK);
+    // the staging buffer holds 3*chunk doubles; larger boxes go z-plane by z-plane
+    if (19 * n <= 3 * chunk) {
+        BGK_HIP(launch_gather_box(L, c->cur, b, c->staging, c->s0));
+        BGK_HIP(hipMemcpyAsync(out, c->staging, sizeof(double) * 19 * n, hipMemcpyDeviceToHost,
+                               c->s0));
+        BGK_HIP(hipStreamSynchronize(c->s0));
+        return BGK_OK;
+    }
+    const long long plane = (long long)(x1 - x0) * (y1 - y0);
+    if (19 * plane > 3 * chunk) return fail(BGK_EINVAL, "box plane too large for get_box");
+    for (int z = z0; z < z1; z++) {
+        const Box bz{x0, x1, y0, y1, z, z + 1};
+        BGK_HIP(launch_gather_box(L, c->cur, bz, c->staging, c->s0));
+        for (int q = 0; q < 19; q++)
+            BGK_HIP(hipMemcpyAsync(out + (size_t)q * n + (size_t)(z - z0) * plane,
+                                   c->staging + (size_t)q * plane, sizeof(double) * plane,
+                                   hipMemcpyDeviceToHost, c->s0));
+        BGK_HIP(hipStreamSynchronize(c->s0));
+    }
+    return BGK_OK;
+}
+
 int bgk_checksum(bgk_ctx *c, uint64_t *out) {
     int r = check_ctx(c);
     if (r) return r;

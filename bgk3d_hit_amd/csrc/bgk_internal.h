@@ -136,6 +136,9 @@ hipError_t launch_init_equilibrium(const Lattice &L, int copy, const double *u, 
 // (partials: nblocks 64-bit words, out: one 64-bit word, both on the device)
 hipError_t launch_checksum(const Lattice &L, int cur, void *partials, int nblocks, void *out,
                            hipStream_t s);
+// populations of box b of copy `cur` into out[19][box sites] (doubles, device)
+hipError_t launch_gather_box(const Lattice &L, int cur, const Box &b, double *out,
+                             hipStream_t s);
 // zero the ghost planes of the slab layout of one copy
 hipError_t launch_fill_zero(void *p, size_t bytes, hipStream_t s);
 

@@ -418,6 +418,34 @@ static hipError_t init_eq_t(const Lattice &L, int copy, const double *u, long lo
     return hipGetLastError();
 }
 
+// ------------------------------------------------------------ box gather
+// populations of the box [x0,x1) x [y0,y1) x [z0,z1) as doubles,
+// out[q][z][y][x] (x fastest), for host-side probes and slices
+template <typename S, typename C>
+__global__ __launch_bounds__(256) void k_gather_box(Pops<S> P, Box b, int nx, int ny, int zoff,
+                                                    double *out) {
+    const long long bx = b.x1 - b.x0, by = b.y1 - b.y0, bz = b.z1 - b.z0;
+    const long long n = bx * by * bz;
+    const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const int x = b.x0 + (int)(i % bx), y = b.y0 + (int)((i / bx) % by),
+              z = b.z0 + (int)(i / (bx * by));
+    const size_t site = (size_t)x + (size_t)nx * (y + (size_t)ny * (z + zoff));
+#pragma unroll
+    for (int q = 0; q < 19; q++)
+        out[(size_t)q * n + i] = (double)Storage<S>::template load<C>(P.p[q][site]);
+}
+
+template <typename S, typename C>
+static hipError_t gather_box_t(const Lattice &L, int cur, const Box &b, double *out,
+                               hipStream_t s) {
+    const long long n = b.sites();
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL((k_gather_box<S, C>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s,
+                       pops_of<S>(L, cur), b, L.nx, L.ny, L.zoff, out);
+    return hipGetLastError();
+}
+
 // ------------------------------------------------------------ kind dispatch
 #define BGK_DISPATCH(storage, FN, ...)                                   \
     switch (storage) {                                                   \
@@ -474,6 +502,10 @@ hipError_t launch_init_equilibrium(const Lattice &L, int copy, const double *u, 
 hipError_t launch_checksum(const Lattice &L, int cur, void *partials, int nblocks, void *out,
                            hipStream_t s) {
     BGK_DISPATCH_S(L.storage, checksum_t, L, cur, partials, nblocks, out, s);
+}
+hipError_t launch_gather_box(const Lattice &L, int cur, const Box &b, double *out,
+                             hipStream_t s) {
+    BGK_DISPATCH(L.storage, gather_box_t, L, cur, b, out, s);
 }
 hipError_t launch_fill_zero(void *p, size_t bytes, hipStream_t s) {
     return hipMemsetAsync(p, 0, bytes, s);

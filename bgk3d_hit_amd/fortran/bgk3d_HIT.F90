@@ -19,6 +19,12 @@
 !   diag    = 'refcompat' (reproduce the reference's diagnostics, stale
 !             halos and the ztot column) or 'consistent'
 !   omega   = relaxation frequency; when > 0 it replaces 2/(6 svisc + 1)
+! and gives meaning to the reference's unused restart keys: itsave > 0 writes
+! restart.bgk every itsave steps; irestart = 1 starts from restart.bgk.
+! Observability as the reference's diagnostic: probe_global (fort.88, the
+! populations of one site, SRC/probe_global.F90:55-95) every icheck steps
+! and the VTK velocity slices tec_{xy,xz,yz}.RRRRR.TTTTTTTT.vtk
+! (SRC/vtk_xy.binary.F90) every ivtim steps and at step 0.
 module bgk_host
   use iso_c_binding
   use bgk_capi
@@ -60,14 +66,19 @@ program bgk_HIT
   call outdat
   call alloca
   call init_fields
+  if (irestart == 1) call read_restart(itstart)
   call system_clock(c1)
   time_init = real(c1 - c0, dp) / real(count_rate, dp)
 
+  call vtk_slices(itstart - 1)                    ! SRC/bgk3d_HIT.F90:118-120
   call system_clock(e0)
   do itime = itstart, itfin                      ! SRC/bgk3d_HIT.F90:127-146
      call boundaries(itime)
      call col_MC(itime)
      call diagnostic(itime)
+     if (itsave > 0) then
+        if (mod(itime, itsave) == 0) call write_restart(itime)
+     end if
   end do
   call bgk_check(bgk_synchronize(ctx), 'bgk_synchronize')
   call system_clock(e1)
@@ -256,6 +267,8 @@ subroutine diagnostic(itime)
      call system_clock(c0)
      call diagno(itime)
      call dissipation(itime)
+     call probe_global(itime, lz / 2, ly / 2, lz / 2, 88)  ! SRC/diagnostic.F90:50
+     if (mod(itime, ivtim) == 0) call vtk_slices(itime)
      call system_clock(c1)
      time_dg = time_dg + real(c1 - c0, dp) / real(count_rate, dp)
   end if
@@ -319,6 +332,153 @@ subroutine dissipation(itime)
   flush(73)
 1004 format(i8, 4(e14.6, 1x))
 end subroutine dissipation
+
+! SRC/probe_global.F90:55-95: the 19 populations of global site (i0,j0,k0)
+subroutine probe_global(itime, ip, jp, kp, unit)
+  use bgk_host
+  implicit none
+  integer, intent(in) :: itime, ip, jp, kp, unit
+  real(c_double) :: f(19)
+  if (ip < 1 .or. ip > l .or. jp < 1 .or. jp > m .or. kp < 1 .or. kp > n) return
+  call bgk_check(bgk_get_box(ctx, int(ip - 1, c_int), int(ip, c_int), int(jp - 1, c_int), &
+       int(jp, c_int), int(kp - 1, c_int), int(kp, c_int), f), 'bgk_get_box')
+  write(unit, *) itime
+  if (kind_c == BGK_STORE_F64 .or. kind_c == BGK_STORE_F32_MIXED) then
+     write(unit, *) f(1), f(2), f(3), f(4)
+     write(unit, *) f(5), f(6), f(7), f(8)
+     write(unit, *) f(9), f(10), f(11), f(12)
+     write(unit, *) f(13), f(14), f(15), f(16)
+     write(unit, *) f(17), f(18), f(19)
+  else
+     write(unit, *) real(f(1:4), sp)
+     write(unit, *) real(f(5:8), sp)
+     write(unit, *) real(f(9:12), sp)
+     write(unit, *) real(f(13:16), sp)
+     write(unit, *) real(f(17:19), sp)
+  end if
+  write(unit, *) " "
+  flush(unit)
+end subroutine probe_global
+
+! SRC/vtk_xy.binary.F90, vtk_xz, vtk_yz at the reference's planes n/2, m/2, l/2
+subroutine vtk_slices(itime)
+  use bgk_host
+  implicit none
+  integer, intent(in) :: itime
+  if (ivtim <= 0 .or. ivtim >= 1000000000) return
+  call vtk_plane(itime, 'xy', 1, l, 1, m, n / 2, n / 2)
+  call vtk_plane(itime, 'xz', 1, l, m / 2, m / 2, 1, n)
+  call vtk_plane(itime, 'yz', l / 2, l / 2, 1, m, 1, n)
+end subroutine vtk_slices
+
+! one legacy-VTK STRUCTURED_POINTS slice: ASCII header, then big-endian
+! float32 velocity triplets u/den, w/den, v/den with x fastest
+subroutine vtk_plane(itime, tag, xa, xb, ya, yb, za, zb)
+  use bgk_host
+  implicit none
+  integer, intent(in) :: itime, xa, xb, ya, yb, za, zb
+  character(len=2), intent(in) :: tag
+  character(len=25) :: file_name
+  real(c_double), allocatable :: f(:, :)
+  real(sp) :: u, w, v, den
+  real(dp) :: ud, wd, vd, dend
+  integer :: ni, nj, nk, np, p
+  if (xa < 1 .or. ya < 1 .or. za < 1) return
+  ni = xb - xa + 1
+  nj = yb - ya + 1
+  nk = zb - za + 1
+  np = ni * nj * nk
+  allocate(f(np, 19))
+  call bgk_check(bgk_get_box(ctx, int(xa - 1, c_int), int(xb, c_int), int(ya - 1, c_int), &
+       int(yb, c_int), int(za - 1, c_int), int(zb, c_int), f), 'bgk_get_box')
+  file_name = 'tec_' // tag // '.yyyyy.xxxxxxxx.vtk'
+  write(file_name(8:12), '(i5.5)') 0
+  write(file_name(14:21), '(i8.8)') itime
+  open(52, file=file_name, status='unknown')
+  write(52, '(A26)') '# vtk DataFile Version 2.0'
+  write(52, '(A5)') 'Campo'
+  write(52, '(A6)') 'BINARY'
+  write(52, '(A25)') 'DATASET STRUCTURED_POINTS'
+  write(52, '(A11,I10,A1,I10,A1,I10)') 'DIMENSIONS ', ni, ' ', nj, ' ', nk
+  write(52, '(A7,I10,A1,I10,A1,I10)') 'ORIGIN ', xa, ' ', ya, ' ', za
+  write(52, '(A8,I10,A1,I10,A1,I10)') 'SPACING ', 1, ' ', 1, ' ', 1
+  write(52, '(A10,I10)') 'POINT_DATA ', np
+  write(52, '(A23)') 'VECTORS velocity float'
+  close(52)
+  open(52, file=file_name, status='old', position='append', form='unformatted', &
+       access='stream', convert='big_endian')
+  do p = 1, np
+     if (kind_c == BGK_STORE_F64) then
+        ! DOUBLE_P build: sums in double, rounded to the float32 output
+        ud = f(p,1) + f(p,2) + f(p,3) + f(p,4) + f(p,5) - f(p,10) - f(p,11) - f(p,12) - &
+             f(p,13) - f(p,14)
+        wd = f(p,3) + f(p,7) + f(p,8) + f(p,9) + f(p,12) - f(p,1) - f(p,10) - f(p,16) - &
+             f(p,17) - f(p,18)
+        vd = f(p,4) + f(p,6) + f(p,7) + f(p,13) + f(p,18) - f(p,2) - f(p,9) - f(p,11) - &
+             f(p,15) - f(p,16)
+        dend = sum(f(p, :)) + 1.0_dp
+        u = real(ud, sp); w = real(wd, sp); v = real(vd, sp); den = real(dend, sp)
+     else
+        ! single-precision storage: sums of the stored values in float32
+        u = real(f(p,1), sp) + real(f(p,2), sp) + real(f(p,3), sp) + real(f(p,4), sp) + &
+            real(f(p,5), sp) - real(f(p,10), sp) - real(f(p,11), sp) - real(f(p,12), sp) - &
+            real(f(p,13), sp) - real(f(p,14), sp)
+        w = real(f(p,3), sp) + real(f(p,7), sp) + real(f(p,8), sp) + real(f(p,9), sp) + &
+            real(f(p,12), sp) - real(f(p,1), sp) - real(f(p,10), sp) - real(f(p,16), sp) - &
+            real(f(p,17), sp) - real(f(p,18), sp)
+        v = real(f(p,4), sp) + real(f(p,6), sp) + real(f(p,7), sp) + real(f(p,13), sp) + &
+            real(f(p,18), sp) - real(f(p,2), sp) - real(f(p,9), sp) - real(f(p,11), sp) - &
+            real(f(p,15), sp) - real(f(p,16), sp)
+        den = sum(real(f(p, :), sp)) + 1.0_sp
+     end if
+     write(52) u / den, w / den, v / den
+  end do
+  close(52)
+  deallocate(f)
+end subroutine vtk_plane
+
+! restart file: 'BGK3DRST', lx, ly, lz, itime, storage kind, then the 19
+! population planes as doubles (exact for every storage kind)
+subroutine write_restart(itime)
+  use bgk_host
+  implicit none
+  integer, intent(in) :: itime
+  real(c_double), allocatable :: f(:)
+  allocate(f(19_8 * int(l, 8) * int(m, 8) * int(n, 8)))
+  call bgk_check(bgk_get_populations(ctx, f), 'bgk_get_populations')
+  open(60, file='restart.bgk', status='replace', form='unformatted', access='stream')
+  write(60) 'BGK3DRST', int(lx, 4), int(ly, 4), int(lz, 4), int(itime, 4), int(kind_c, 4)
+  write(60) f
+  close(60)
+  deallocate(f)
+end subroutine write_restart
+
+subroutine read_restart(itstart)
+  use bgk_host
+  implicit none
+  integer, intent(out) :: itstart
+  real(c_double), allocatable :: f(:)
+  character(len=8) :: magic
+  integer(4) :: rx, ry, rz, rit, rkind
+  integer :: ios
+  open(60, file='restart.bgk', status='old', form='unformatted', access='stream', iostat=ios)
+  if (ios /= 0) then
+     write(6, *) 'ERROR: irestart=1 but restart.bgk cannot be opened'
+     error stop 1
+  end if
+  read(60) magic, rx, ry, rz, rit, rkind
+  if (magic /= 'BGK3DRST' .or. rx /= lx .or. ry /= ly .or. rz /= lz) then
+     write(6, *) 'ERROR: restart.bgk does not match the lattice', rx, ry, rz
+     error stop 1
+  end if
+  allocate(f(19_8 * int(l, 8) * int(m, 8) * int(n, 8)))
+  read(60) f
+  close(60)
+  call bgk_check(bgk_set_populations(ctx, f), 'bgk_set_populations')
+  deallocate(f)
+  itstart = rit + 1
+  write(6, *) 'INFO: restarted from step ', rit
+end subroutine read_restart
 
 ! SRC/finalize.F90:44-103
 subroutine finalize(itstart)

@@ -137,6 +137,14 @@ module bgk_capi
        real(c_double), intent(out) :: out(2)
      end function bgk_dissipation
 
+     integer(c_int) function bgk_get_box(ctx, x0, x1, y0, y1, z0, z1, out) &
+          bind(c, name='bgk_get_box')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: x0, x1, y0, y1, z0, z1
+       real(c_double), intent(out) :: out(*)
+     end function bgk_get_box
+
      integer(c_int) function bgk_checksum(ctx, out) bind(c, name='bgk_checksum')
        import :: c_int, c_ptr, c_int64_t
        type(c_ptr), value :: ctx

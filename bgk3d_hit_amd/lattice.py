@@ -158,6 +158,12 @@ class Lattice:
                                           else L.DIAG_CONSISTENT, _dptr(out)))
         return out
 
+    def get_box(self, x0, x1, y0, y1, z0, z1):
+        """Populations of a sub-box, shape (19, z1-z0, y1-y0, x1-x0)."""
+        out = np.empty((19, z1 - z0, y1 - y0, x1 - x0))
+        L.check(self._lib.bgk_get_box(self._h, x0, x1, y0, y1, z0, z1, _dptr(out)))
+        return out
+
     def checksum(self):
         """64-bit order-independent hash of the stored populations."""
         v = ctypes.c_uint64(0)

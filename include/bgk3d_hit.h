@@ -147,6 +147,13 @@ int bgk_diagno(bgk_ctx *ctx, int mode, double out[5]);
  * reference's loctot(1..2), before its output normalisation).              */
 int bgk_dissipation(bgk_ctx *ctx, double svisc, int mode, double out[2]);
 
+/* Populations of the box [x0,x1) x [y0,y1) x [z0,z1) (0-based, interior
+ * sites) into out: 19 planes of (x1-x0)(y1-y0)(z1-z0) doubles, x fastest.
+ * Device-side gather replacing the host-array reads of probe_global
+ * (SRC/probe_global.F90:55-95, one site) and of the VTK slice writers
+ * (SRC/vtk_xy.binary.F90:57-99 and the xz/yz/3d variants, one plane).       */
+int bgk_get_box(bgk_ctx *ctx, int x0, int x1, int y0, int y1, int z0, int z1, double *out);
+
 /* Order-independent 64-bit hash of the raw stored bits of the current
  * interior populations (equal hashes <=> bit-identical states, up to hash
  * collisions).  No reference counterpart: the reference compares states

@@ -75,3 +75,57 @@ def test_driver_tgv_reference_deck_shape(gpu, tmp_path):
     assert d.shape[0] == 2
     assert np.abs(d[:, 4] - 1).max() < 1e-5          # mean rho (FP32 build)
     assert (d[1, 5] < d[0, 5]) and d[1, 5] > 0         # decaying kinetic energy
+
+
+def read_vtk(path):
+    raw = open(path, "rb").read()
+    marker = b"VECTORS velocity float\n"  # (A23): right-justified, as the reference writes it
+    i = raw.index(marker) + len(marker)
+    header = raw[:i].decode().splitlines()
+    data = np.frombuffer(raw[i:], dtype=">f4").reshape(-1, 3)
+    return header, data
+
+
+@pytest.mark.gpu
+def test_driver_probe_vtk_and_restart(gpu, tmp_path):
+    """probe_global (fort.88) and the VTK slices against the oracle state, and
+    restart: 20 steps + restart + 20 steps == 40 steps, bit for bit."""
+    n, svisc = 24, 0.01
+    base = (f"lx={n}\nly={n}\nlz={n}\nsvisc={svisc}\ninit='hit'\nk0=3.0\nurms=0.03\n"
+            f"seed=7\nstorage='f64'\ndiag='consistent'\n")
+    print("[gpu-case] fortran driver probe/vtk/restart", flush=True)
+    a = tmp_path / "a"
+    a.mkdir()
+    run_driver(a, f"&parameters\n{base}itfin=40\nicheck=20\nivtim=40\nitsave=20\n/\n")
+    b = tmp_path / "b"
+    b.mkdir()
+    run_driver(b, f"&parameters\n{base}itfin=20\nicheck=20\nitsave=20\n/\n")
+    run_driver(b, f"&parameters\n{base}itfin=40\nicheck=20\nitsave=20\nirestart=1\n/\n")
+    ra, rb = (a / "restart.bgk").read_bytes(), (b / "restart.bgk").read_bytes()
+    assert ra[:8] == b"BGK3DRST" and ra == rb
+    # oracle state at step 40
+    omega = 2.0 / (6.0 * svisc + 1.0)
+    u = bgk.hit_velocity(n, n, n, 3.0, 0.03, 7)
+    o = oracle.OracleLattice(n, n, n, omega=omega, kind="f64")
+    o.init_equilibrium(u)
+    o.step(40)
+    f = o.get_populations()
+    assert np.array_equal(np.frombuffer(ra[28:], dtype=np.float64).reshape(f.shape), f)
+    # fort.88: itime, then the 19 populations of site (lz/2, ly/2, lz/2) (1-based)
+    vals = (a / "fort.88").read_text().split()
+    i = len(vals) - 19
+    assert int(vals[i - 1]) == 40
+    site = f[:, n // 2 - 1, n // 2 - 1, n // 2 - 1]
+    assert np.allclose(np.array(vals[i:], dtype=float), site, rtol=1e-14, atol=0)
+    # xy slice at z = n/2 (1-based): float32 of (j/rho) from double sums
+    header, data = read_vtk(a / "tec_xy.00000.00000040.vtk")
+    assert header[4].split() == ["DIMENSIONS", str(n), str(n), "1"]
+    g = f[:, n // 2 - 1]
+    C = np.array([(1, -1, 0), (1, 0, -1), (1, 1, 0), (1, 0, 1), (1, 0, 0), (0, 0, 1), (0, 1, 1),
+                  (0, 1, 0), (0, 1, -1), (-1, -1, 0), (-1, 0, -1), (-1, 1, 0), (-1, 0, 1),
+                  (-1, 0, 0), (0, 0, -1), (0, -1, -1), (0, -1, 0), (0, -1, 1), (0, 0, 0)])
+    rho = g.sum(axis=0) + 1.0
+    j = np.einsum("qa,qyx->ayx", C.astype(float), g)
+    exp = (j.astype(np.float32) / rho.astype(np.float32)).reshape(3, -1).T
+    assert np.allclose(data, exp, rtol=2e-6, atol=1e-9)
+    assert (a / "tec_xz.00000.00000040.vtk").exists() and (a / "tec_yz.00000.00000000.vtk").exists()

@@ -296,3 +296,16 @@ def test_config3_slab_1024x1024x256_bitwise(gpu):
     s.step(10)
     assert s.checksum() == cp
     assert np.array_equal(s.diagno(), dp_)
+
+
+def test_get_box_matches_full_download(gpu):
+    shape = (40, 36, 30)
+    g = bgk.Lattice(*shape, omega=1.9, storage="f16")
+    g.init_hit(3.0, 0.03, 11)
+    g.step(3)
+    f = g.get_populations()
+    for (x0, x1, y0, y1, z0, z1) in [(5, 6, 7, 8, 9, 10), (0, 40, 0, 36, 15, 16),
+                                     (3, 17, 0, 36, 2, 29), (0, 40, 18, 19, 0, 30)]:
+        assert np.array_equal(g.get_box(x0, x1, y0, y1, z0, z1), f[:, z0:z1, y0:y1, x0:x1])
+    with pytest.raises(bgk.BGKError):
+        g.get_box(0, 41, 0, 1, 0, 1)
