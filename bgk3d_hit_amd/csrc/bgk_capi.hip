@@ -144,14 +144,16 @@ static int collide_boxes(bgk_ctx *c, const std::vector<Box> &boxes, hipStream_t 
 }
 
 // slab halo exchange of copy `cur` on stream s: pack the 5 crossing
-// populations of both z faces, deliver the messages (periodic self-exchange:
-// this rank is its own up and down neighbour), unpack into the ghost planes
+// populations of both z faces into the message buffer, deliver, unpack into
+// the ghost planes.  In a periodic self-exchange this rank is its own up and
+// down neighbour, so the delivered message is the packed buffer itself; a
+// multi-GPU build sends halo_send to the neighbours' halo_recv here
+// (ncclSend / ncclRecv) and unpacks from halo_recv.
 static int halo_exchange(bgk_ctx *c, hipStream_t s) {
     hipEvent_t ev;
     BGK_HIP(timing_begin(c, TCLS_HALO, s, &ev));
     BGK_HIP(launch_pack_z(c->lat, c->cur, c->halo_send, s));
-    BGK_HIP(hipMemcpyAsync(c->halo_recv, c->halo_send, c->halo_bytes, hipMemcpyDeviceToDevice, s));
-    BGK_HIP(launch_unpack_z(c->lat, c->cur, c->halo_recv, s));
+    BGK_HIP(launch_unpack_z(c->lat, c->cur, c->halo_send, s));
     BGK_HIP(timing_end(c, TCLS_HALO, s, ev));
     return BGK_OK;
 }
@@ -329,9 +331,7 @@ int bgk_create(int lx, int ly, int lz, int storage, int layout, int flags, doubl
         c->halo_bytes = (size_t)2 * 5 * lx * ly * es;
         if ((e = hipMalloc(&c->halo_send, c->halo_bytes)) != hipSuccess)
             return bail(hip_fail(e, "hipMalloc(halo)"));
-        if ((e = hipMalloc(&c->halo_recv, c->halo_bytes)) != hipSuccess)
-            return bail(hip_fail(e, "hipMalloc(halo)"));
-        c->bytes += 2 * c->halo_bytes;
+        c->bytes += c->halo_bytes;
     }
     if (flags & BGK_FLAG_REFCOMPAT_HALOS) {
         c->gcache_bytes =
@@ -376,7 +376,6 @@ int bgk_destroy(bgk_ctx *c) {
     if (c->s1) (void)hipStreamDestroy(c->s1);
     (void)hipFree(c->pop_alloc);
     (void)hipFree(c->halo_send);
-    (void)hipFree(c->halo_recv);
     (void)hipFree(c->gcache[0]);
     (void)hipFree(c->gcache[1]);
     (void)hipFree(c->red_partials);

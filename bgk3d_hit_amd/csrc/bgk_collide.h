@@ -20,13 +20,34 @@ static Pops<S> pops_of(const Lattice &L, int copy) {
     return P;
 }
 
-// Launch geometry of a box of sites in 64 x 4 tiles (one z plane deep).
+// Tile (workgroup) shape per storage kind, from a sweep of 32x8 .. 256x1
+// tiles at 512^3 and 1024^3: a wider x extent leaves fewer partially used
+// 128-B lines at the tile edges for the 10 x-shifted populations.
+template <typename S>
+struct Tile {
+    static constexpr int X = 128, Y = 4;  // float
+};
+template <>
+struct Tile<double> {
+    static constexpr int X = 128, Y = 2;
+};
+template <>
+struct Tile<half_t> {
+    static constexpr int X = 256, Y = 1;
+};
+template <>
+struct Tile<bf16_t> {
+    static constexpr int X = 256, Y = 1;
+};
+
+// Launch geometry of a box of sites in TX x TY tiles (one z plane deep).
 struct Geo {
     int nx, ny, nz, nxy, zoff, periodic_z;
     int tiles_x, tiles_y;
     int x0, x1, y0, y1, z0;
 };
 
+template <typename S>
 static Geo make_geo(const Lattice &L, const Box &b, int *ntiles) {
     Geo g;
     g.nx = L.nx;
@@ -35,8 +56,8 @@ static Geo make_geo(const Lattice &L, const Box &b, int *ntiles) {
     g.nxy = L.nx * L.ny;
     g.zoff = L.zoff;
     g.periodic_z = (L.layout == BGK_LAYOUT_PERIODIC);
-    g.tiles_x = (b.x1 - b.x0 + 63) / 64;
-    g.tiles_y = (b.y1 - b.y0 + 3) / 4;
+    g.tiles_x = (b.x1 - b.x0 + Tile<S>::X - 1) / Tile<S>::X;
+    g.tiles_y = (b.y1 - b.y0 + Tile<S>::Y - 1) / Tile<S>::Y;
     g.x0 = b.x0;
     g.x1 = b.x1;
     g.y0 = b.y0;
@@ -59,7 +80,7 @@ __device__ __forceinline__ int xcd_remap(int b, int G) {
 // TAPE: also write the pulled-moment velocity u = j/rho of every site to
 // tape[0..2][site] (the adjoint's record of the forward pass).
 template <typename S, typename C, bool TAPE = false>
-__global__ __launch_bounds__(256) void k_collide(Pops<S> src, Pops<S> dst, Geo g, C omega,
+__global__ __launch_bounds__(Tile<S>::X * Tile<S>::Y) void k_collide(Pops<S> src, Pops<S> dst, Geo g, C omega,
                                                  C cte1, C *__restrict__ tape = nullptr,
                                                  long long tape_ld = 0) {
     const int t = xcd_remap((int)blockIdx.x, (int)gridDim.x);
@@ -67,8 +88,8 @@ __global__ __launch_bounds__(256) void k_collide(Pops<S> src, Pops<S> dst, Geo g
     const int r = t / g.tiles_x;
     const int ty = r % g.tiles_y;
     const int tz = r / g.tiles_y;
-    const int x = g.x0 + tx * 64 + (int)threadIdx.x;
-    const int y = g.y0 + ty * 4 + (int)threadIdx.y;
+    const int x = g.x0 + tx * Tile<S>::X + (int)threadIdx.x;
+    const int y = g.y0 + ty * Tile<S>::Y + (int)threadIdx.y;
     const int z = g.z0 + tz;
     if (x >= g.x1 || y >= g.y1) return;
 

@@ -66,8 +66,8 @@ struct bgk_ctx {
 
     void *pop_alloc = nullptr;  // one allocation holding 37 population planes
 
-    // slab halo messages: [face 0 = z+ (plane nz), 1 = z- (plane 1)][5][nx*ny]
-    void *halo_send = nullptr, *halo_recv = nullptr;
+    // slab halo message: [face 0 = z+ (plane nz), 1 = z- (plane 1)][5][nx*ny]
+    void *halo_send = nullptr;
     size_t halo_bytes = 0;
 
     // stale-halo images (reference ghost semantics), per copy:

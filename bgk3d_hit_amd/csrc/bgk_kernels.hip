@@ -23,10 +23,10 @@ static hipError_t collide_t(const Lattice &L, int src, const Box &box, double om
                             hipStream_t s) {
     if (box.sites() <= 0) return hipSuccess;
     int ntiles;
-    Geo g = make_geo(L, box, &ntiles);
+    Geo g = make_geo<S>(L, box, &ntiles);
     Pops<S> a = pops_of<S>(L, src), b = pops_of<S>(L, 1 - src);
-    hipLaunchKernelGGL((k_collide<S, C>), dim3(ntiles), dim3(64, 4), 0, s, a, b, g, (C)omega,
-                       (C)(L.shifted ? 1.0 : 0.0));
+    hipLaunchKernelGGL((k_collide<S, C>), dim3(ntiles), dim3(Tile<S>::X, Tile<S>::Y), 0, s, a, b,
+                       g, (C)omega, (C)(L.shifted ? 1.0 : 0.0));
     return hipGetLastError();
 }
 
