@@ -218,7 +218,7 @@ static int step_slab(bgk_ctx *c, int nsteps) {
 }
 
 int reduce_ke(bgk_ctx *c, int copy, double *J) {
-    BGK_HIP(launch_diagno(c->lat, copy, c->red_partials, c->red_blocks, c->red_out, c->s0));
+    BGK_HIP(launch_diagno(c->lat, copy, c->red_partials, c->red_out, c->s0));
     BGK_HIP(hipMemcpyAsync(c->red_host, c->red_out, 8 * sizeof(double), hipMemcpyDeviceToHost,
                            c->s0));
     BGK_HIP(hipStreamSynchronize(c->s0));
@@ -345,8 +345,9 @@ int bgk_create(int lx, int ly, int lz, int storage, int layout, int flags, doubl
         c->bytes += 2 * c->gcache_bytes;
     }
     long long nb = (L.nsite + 255) / 256;
-    c->red_blocks = (int)std::min<long long>(std::max<long long>(nb, 1), 2048);
-    if ((e = hipMalloc(&c->red_partials, sizeof(double) * 8 * c->red_blocks)) != hipSuccess)
+    c->red_blocks = (int)std::min<long long>(std::max<long long>(nb, 1), 2048);  // checksum
+    const int nparts = std::max(c->red_blocks, reduction_parts(L));
+    if ((e = hipMalloc(&c->red_partials, sizeof(double) * 8 * nparts)) != hipSuccess)
         return bail(hip_fail(e, "hipMalloc(reduction)"));
     if ((e = hipMalloc(&c->red_out, sizeof(double) * 8)) != hipSuccess)
         return bail(hip_fail(e, "hipMalloc(reduction)"));
@@ -356,7 +357,7 @@ int bgk_create(int lx, int ly, int lz, int storage, int layout, int flags, doubl
     const long long chunk = std::min(L.nsite, CHUNK);
     if ((e = hipMalloc(&c->staging, sizeof(double) * 3 * chunk)) != hipSuccess)
         return bail(hip_fail(e, "hipMalloc(staging)"));
-    c->bytes += sizeof(double) * (3 * chunk + 8 * c->red_blocks + 8);
+    c->bytes += sizeof(double) * (3 * chunk + 8 * (size_t)nparts + 8);
     if ((e = hipStreamSynchronize(c->s0)) != hipSuccess)
         return bail(hip_fail(e, "hipStreamSynchronize"));
     (void)rc;
@@ -548,7 +549,7 @@ int bgk_diagno(bgk_ctx *c, int mode, double out[5]) {
     const Lattice &L = c->lat;
     hipEvent_t ev;
     BGK_HIP(timing_begin(c, TCLS_REDUCE, c->s0, &ev));
-    BGK_HIP(launch_diagno(L, c->cur, c->red_partials, c->red_blocks, c->red_out, c->s0));
+    BGK_HIP(launch_diagno(L, c->cur, c->red_partials, c->red_out, c->s0));
     BGK_HIP(timing_end(c, TCLS_REDUCE, c->s0, ev));
     BGK_HIP(hipMemcpyAsync(c->red_host, c->red_out, 8 * sizeof(double), hipMemcpyDeviceToHost,
                            c->s0));
@@ -589,8 +590,7 @@ int bgk_dissipation(bgk_ctx *c, double svisc, int mode, double out[2]) {
     }
     hipEvent_t ev;
     BGK_HIP(timing_begin(c, TCLS_REDUCE, c->s0, &ev));
-    BGK_HIP(launch_dissipation(L, c->cur, gc, svisc, c->red_partials, c->red_blocks, c->red_out,
-                               c->s0));
+    BGK_HIP(launch_dissipation(L, c->cur, gc, svisc, c->red_partials, c->red_out, c->s0));
     BGK_HIP(timing_end(c, TCLS_REDUCE, c->s0, ev));
     BGK_HIP(hipMemcpyAsync(c->red_host, c->red_out, 8 * sizeof(double), hipMemcpyDeviceToHost,
                            c->s0));

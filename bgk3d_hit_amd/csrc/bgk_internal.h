@@ -76,7 +76,7 @@ struct bgk_ctx {
     size_t gcache_bytes = 0;
 
     // reductions
-    double *red_partials = nullptr;  // [max_blocks][8]
+    double *red_partials = nullptr;  // [max(red_blocks, reduction_parts)][8]
     double *red_out = nullptr;       // [8] device
     double *red_host = nullptr;      // [8] pinned host
     int red_blocks = 0;
@@ -113,10 +113,12 @@ hipError_t launch_collide(const Lattice &L, int src, const Box &box, double omeg
 // reductions: at-site (diagno) and pulled (dissipation)
 //   mode 0: pulled gather with periodic wrap (or slab ghosts as stored)
 //   mode 1: pulled gather reading face images from gcache (periodic layout)
-hipError_t launch_diagno(const Lattice &L, int cur, double *partials, int nblocks, double *out,
+//   partials: reduction_parts(L) slots of 8 doubles
+int reduction_parts(const Lattice &L);
+hipError_t launch_diagno(const Lattice &L, int cur, double *partials, double *out,
                          hipStream_t s);
 hipError_t launch_dissipation(const Lattice &L, int cur, const void *gcache, double svisc,
-                              double *partials, int nblocks, double *out, hipStream_t s);
+                              double *partials, double *out, hipStream_t s);
 // slab halo path
 hipError_t launch_pack_z(const Lattice &L, int cur, void *send, hipStream_t s);
 hipError_t launch_unpack_z(const Lattice &L, int cur, const void *recv, hipStream_t s);

@@ -14,6 +14,8 @@
 //
 // All reductions accumulate in FP64 with a fixed grid and a fixed
 // combination order, so they are deterministic run to run.
+#include <algorithm>
+
 #include "bgk_collide.h"
 
 namespace bgk {
@@ -33,8 +35,11 @@ static hipError_t collide_t(const Lattice &L, int src, const Box &box, double om
 // --------------------------------------------------------------- reductions
 constexpr int RED_THREADS = 256;
 constexpr int RED_NV = 8;
+constexpr int FIN_THREADS = 1024;              // threads of the final combine
+constexpr long long RED_TARGET_PARTS = 16384;  // partial sums per reduction
 
-// Reduce NV doubles over the block and write them to out[0..NV).
+// Reduce NV doubles over the block (any size up to 1024 threads, all threads
+// taking part) and write them to out[0..NV).
 template <int NV>
 __device__ __forceinline__ void block_sum_store(double (&v)[NV], double *out) {
 #pragma unroll
@@ -42,140 +47,205 @@ __device__ __forceinline__ void block_sum_store(double (&v)[NV], double *out) {
 #pragma unroll
         for (int k = 0; k < NV; k++) v[k] += __shfl_xor(v[k], off);
     }
-    __shared__ double sm[RED_THREADS / 64][NV];
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    __shared__ double sm[16][NV];
+    const int tid = (int)(threadIdx.x + blockDim.x * threadIdx.y);
+    const int nwaves = (int)(blockDim.x * blockDim.y) / 64;
+    const int lane = tid & 63, wave = tid >> 6;
     if (lane == 0) {
 #pragma unroll
         for (int k = 0; k < NV; k++) sm[wave][k] = v[k];
     }
     __syncthreads();
-    if ((int)threadIdx.x < NV) {
+    if (tid < NV) {
         double s = 0.0;
-#pragma unroll
-        for (int w = 0; w < RED_THREADS / 64; w++) s += sm[w][threadIdx.x];
-        out[threadIdx.x] = s;
+        for (int w = 0; w < nwaves; w++) s += sm[w][tid];
+        out[tid] = s;
     }
 }
 
-__global__ __launch_bounds__(RED_THREADS) void k_finalize(const double *partials, int nblocks,
+__global__ __launch_bounds__(FIN_THREADS) void k_finalize(const double *partials, int nblocks,
                                                           double *out) {
     double v[RED_NV];
 #pragma unroll
     for (int k = 0; k < RED_NV; k++) v[k] = 0.0;
-    for (int b = threadIdx.x; b < nblocks; b += RED_THREADS) {
+    for (int b = threadIdx.x; b < nblocks; b += FIN_THREADS) {
 #pragma unroll
         for (int k = 0; k < RED_NV; k++) v[k] += partials[(size_t)b * RED_NV + k];
     }
     block_sum_store<RED_NV>(v, out);
 }
 
+// Reduction geometry: one block per (x,y) tile of the storage kind's collide
+// tile shape and per chunk of zb consecutive z planes; the partial of a block
+// goes to the slot of its tile, so the combination order is fixed.
+struct RedGeo {
+    int nx, ny, nz, nxy, zoff, periodic_z;
+    int tiles_x, tiles_y, zb, nparts;
+};
+
+template <typename S>
+static RedGeo make_red_geo(const Lattice &L) {
+    RedGeo g;
+    g.nx = L.nx;
+    g.ny = L.ny;
+    g.nz = L.nz;
+    g.nxy = L.nx * L.ny;
+    g.zoff = L.zoff;
+    g.periodic_z = (L.layout == BGK_LAYOUT_PERIODIC);
+    g.tiles_x = (L.nx + Tile<S>::X - 1) / Tile<S>::X;
+    g.tiles_y = (L.ny + Tile<S>::Y - 1) / Tile<S>::Y;
+    const long long txy = (long long)g.tiles_x * g.tiles_y;
+    g.zb = (int)std::max<long long>(1, (txy * L.nz + RED_TARGET_PARTS - 1) / RED_TARGET_PARTS);
+    g.nparts = (int)(txy * ((L.nz + g.zb - 1) / g.zb));
+    return g;
+}
+
+int reduction_parts(const Lattice &L) {
+    switch (L.storage) {
+    case BGK_STORE_F64: return make_red_geo<double>(L).nparts;
+    case BGK_STORE_F32:
+    case BGK_STORE_F32_MIXED: return make_red_geo<float>(L).nparts;
+    case BGK_STORE_F16: return make_red_geo<half_t>(L).nparts;
+    default: return make_red_geo<bf16_t>(L).nparts;
+    }
+}
+
 // at-site moments: v = {sum rho, sum ux, sum uy, sum uz, sum |u|^2/2,
 // uz of the last site (l,m,n)}
 template <typename S, typename C>
-__global__ __launch_bounds__(RED_THREADS) void k_diagno(Pops<S> P, int nx, int ny, int nz,
-                                                        int zoff, C cte1, double *partials) {
+__global__ __launch_bounds__(Tile<S>::X * Tile<S>::Y) void k_diagno(Pops<S> P, RedGeo g, C cte1,
+                                                                    double *partials) {
+    const int t = xcd_remap((int)blockIdx.x, (int)gridDim.x);
+    const int tx = t % g.tiles_x;
+    const int r = t / g.tiles_x;
+    const int ty = r % g.tiles_y;
+    const int zc = r / g.tiles_y;
+    const int x = tx * Tile<S>::X + (int)threadIdx.x;
+    const int y = ty * Tile<S>::Y + (int)threadIdx.y;
     double v[RED_NV];
 #pragma unroll
     for (int k = 0; k < RED_NV; k++) v[k] = 0.0;
-    const long long nsite = (long long)nx * ny * nz;
-    const long long off = (long long)zoff * nx * ny;
-    for (long long i = (long long)blockIdx.x * RED_THREADS + threadIdx.x; i < nsite;
-         i += (long long)gridDim.x * RED_THREADS) {
-        C f[19];
+    if (x < g.nx && y < g.ny) {
+        const int z1 = min(g.nz, (zc + 1) * g.zb);
+        for (int z = zc * g.zb; z < z1; z++) {
+            const size_t site = (size_t)x + (size_t)g.nx * y + (size_t)g.nxy * (z + g.zoff);
+            C f[19];
 #pragma unroll
-        for (int q = 0; q < 19; q++) f[q] = Storage<S>::template load<C>(P.p[q][off + i]);
-        C rho, ux, uy, uz;
-        moments_site<C>(f, cte1, rho, ux, uy, uz);
-        const C sk = (C)0.5 * (ux * ux + uy * uy + uz * uz);
-        v[0] += (double)rho;
-        v[1] += (double)ux;
-        v[2] += (double)uy;
-        v[3] += (double)uz;
-        v[4] += (double)sk;
-        if (i == nsite - 1) v[5] = (double)uz;
+            for (int q = 0; q < 19; q++) f[q] = Storage<S>::template load<C>(P.p[q][site]);
+            C rho, ux, uy, uz;
+            moments_site<C>(f, cte1, rho, ux, uy, uz);
+            const C sk = (C)0.5 * (ux * ux + uy * uy + uz * uz);
+            v[0] += (double)rho;
+            v[1] += (double)ux;
+            v[2] += (double)uy;
+            v[3] += (double)uz;
+            v[4] += (double)sk;
+            if (x == g.nx - 1 && y == g.ny - 1 && z == g.nz - 1) v[5] = (double)uz;
+        }
     }
-    block_sum_store<RED_NV>(v, partials + (size_t)blockIdx.x * RED_NV);
+    block_sum_store<RED_NV>(v, partials + (size_t)t * RED_NV);
 }
 
-// pulled moments + Pi^neq; GC = 0: sources wrap periodically (slab: read
-// the ghost planes as stored), GC = 1: sources outside the box read the
-// recorded face images gx/gy/gz (reference stale-halo semantics)
+// pulled moments + Pi^neq; GC = 0: sources wrap periodically (slab: the
+// ghost planes as stored), GC = 1: sources outside the box read the recorded
+// face images gx/gy/gz (reference stale-halo semantics, periodic layout)
 template <typename S, typename C, int GC>
-__global__ __launch_bounds__(RED_THREADS) void k_dissipation(Pops<S> P, const S *gcache, int nx,
-                                                             int ny, int nz, int zoff,
-                                                             int periodic_z, C cte1, C svisc,
-                                                             double *partials) {
+__global__ __launch_bounds__(Tile<S>::X * Tile<S>::Y) void k_dissipation(Pops<S> P,
+                                                                         const S *gcache,
+                                                                         RedGeo g, C cte1,
+                                                                         C svisc,
+                                                                         double *partials) {
+    const int t = xcd_remap((int)blockIdx.x, (int)gridDim.x);
+    const int tx = t % g.tiles_x;
+    const int r = t / g.tiles_x;
+    const int ty = r % g.tiles_y;
+    const int zc = r / g.tiles_y;
+    const int x = tx * Tile<S>::X + (int)threadIdx.x;
+    const int y = ty * Tile<S>::Y + (int)threadIdx.y;
+    const int nx = g.nx, ny = g.ny, nz = g.nz, nxy = g.nxy;
     double v[RED_NV];
 #pragma unroll
     for (int k = 0; k < RED_NV; k++) v[k] = 0.0;
-    const int nxy = nx * ny;
-    const long long nsite = (long long)nxy * nz;
-    const S *gx = gcache;
-    const S *gy = gcache + (size_t)19 * ny * nz;
-    const S *gz = gy + (size_t)19 * nx * nz;
-    for (long long i = (long long)blockIdx.x * RED_THREADS + threadIdx.x; i < nsite;
-         i += (long long)gridDim.x * RED_THREADS) {
-        const int x = (int)(i % nx);
-        const int y = (int)((i / nx) % ny);
-        const int z = (int)(i / nxy);
-        C f[19];
-#pragma unroll
-        for (int q = 0; q < 19; q++) {
-            int sx = x - cx(q), sy = y - cy(q), sz = z - cz(q);
-            const bool ox = (sx < 0 || sx >= nx), oy = (sy < 0 || sy >= ny);
-            const bool oz = (sz < 0 || sz >= nz);
-            const int wx = sx < 0 ? sx + nx : (sx >= nx ? sx - nx : sx);
-            const int wy = sy < 0 ? sy + ny : (sy >= ny ? sy - ny : sy);
-            const int wz = sz < 0 ? sz + nz : (sz >= nz ? sz - nz : sz);
-            S raw;
-            if (GC && ox) {
-                raw = gx[(size_t)q * ny * nz + wy + (size_t)ny * wz];
-            } else if (GC && oy) {
-                raw = gy[(size_t)q * nx * nz + wx + (size_t)nx * wz];
-            } else if (GC && oz) {
-                raw = gz[(size_t)q * nxy + wx + (size_t)nx * wy];
+    if (x < nx && y < ny) {
+        const int xm = (x == 0) ? nx - 1 : x - 1, xp = (x == nx - 1) ? 0 : x + 1;
+        const int ym = (y == 0) ? ny - 1 : y - 1, yp = (y == ny - 1) ? 0 : y + 1;
+        const int X[3] = {xm, x, xp};
+        const int Y[3] = {ym * nx, y * nx, yp * nx};
+        const int z1 = min(nz, (zc + 1) * g.zb);
+        for (int z = zc * g.zb; z < z1; z++) {
+            const int zs = z + g.zoff;
+            int zm, zp;
+            if (g.periodic_z) {
+                zm = (z == 0) ? nz - 1 : z - 1;
+                zp = (z == nz - 1) ? 0 : z + 1;
             } else {
-                const int zz = periodic_z ? wz : sz + zoff;
-                raw = P.p[q][wx + nx * wy + (size_t)nxy * zz];
+                zm = zs - 1;
+                zp = zs + 1;
             }
-            f[q] = Storage<S>::template load<C>(raw);
+            const int Z[3] = {zm * nxy, zs * nxy, zp * nxy};
+            C f[19];
+#pragma unroll
+            for (int q = 0; q < 19; q++) {
+                S raw = P.p[q][X[1 - cx(q)] + Y[1 - cy(q)] + Z[1 - cz(q)]];
+                if (GC) {
+                    // reference ghosts: a source outside the box reads the face
+                    // image recorded when this copy was last exchanged
+                    const int sx = x - cx(q), sy = y - cy(q), sz = z - cz(q);
+                    const bool ox = (sx < 0 || sx >= nx), oy = (sy < 0 || sy >= ny);
+                    const bool oz = (sz < 0 || sz >= nz);
+                    if (ox | oy | oz) {
+                        const int wx = X[1 - cx(q)], wy = (Y[1 - cy(q)]) / nx;
+                        const int wz = (Z[1 - cz(q)]) / nxy;
+                        const S *gx = gcache;
+                        const S *gy = gcache + (size_t)19 * ny * nz;
+                        const S *gz = gy + (size_t)19 * nx * nz;
+                        if (ox)
+                            raw = gx[(size_t)q * ny * nz + wy + (size_t)ny * wz];
+                        else if (oy)
+                            raw = gy[(size_t)q * nx * nz + wx + (size_t)nx * wz];
+                        else
+                            raw = gz[(size_t)q * nxy + wx + (size_t)nx * wy];
+                    }
+                }
+                f[q] = Storage<S>::template load<C>(raw);
+            }
+            C dsite, tsite;
+            dissipation_site<C>(f, cte1, svisc, dsite, tsite);
+            v[0] += (double)dsite;
+            v[1] += (double)tsite;
         }
-        C dsite, tsite;
-        dissipation_site<C>(f, cte1, svisc, dsite, tsite);
-        v[0] += (double)dsite;
-        v[1] += (double)tsite;
     }
-    block_sum_store<RED_NV>(v, partials + (size_t)blockIdx.x * RED_NV);
+    block_sum_store<RED_NV>(v, partials + (size_t)t * RED_NV);
 }
 
 template <typename S, typename C>
-static hipError_t diagno_t(const Lattice &L, int cur, double *partials, int nblocks, double *out,
+static hipError_t diagno_t(const Lattice &L, int cur, double *partials, double *out,
                            hipStream_t s) {
-    hipLaunchKernelGGL((k_diagno<S, C>), dim3(nblocks), dim3(RED_THREADS), 0, s,
-                       pops_of<S>(L, cur), L.nx, L.ny, L.nz, L.zoff, (C)(L.shifted ? 1.0 : 0.0),
-                       partials);
+    const RedGeo g = make_red_geo<S>(L);
+    hipLaunchKernelGGL((k_diagno<S, C>), dim3(g.nparts), dim3(Tile<S>::X, Tile<S>::Y), 0, s,
+                       pops_of<S>(L, cur), g, (C)(L.shifted ? 1.0 : 0.0), partials);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(RED_THREADS), 0, s, partials, nblocks, out);
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(FIN_THREADS), 0, s, partials, g.nparts, out);
     return hipGetLastError();
 }
 
 template <typename S, typename C>
 static hipError_t dissipation_t(const Lattice &L, int cur, const void *gcache, double svisc,
-                                double *partials, int nblocks, double *out, hipStream_t s) {
+                                double *partials, double *out, hipStream_t s) {
+    const RedGeo g = make_red_geo<S>(L);
     const C cte1 = (C)(L.shifted ? 1.0 : 0.0);
-    const int pz = (L.layout == BGK_LAYOUT_PERIODIC);
     if (gcache)
-        hipLaunchKernelGGL((k_dissipation<S, C, 1>), dim3(nblocks), dim3(RED_THREADS), 0, s,
-                           pops_of<S>(L, cur), (const S *)gcache, L.nx, L.ny, L.nz, L.zoff, pz,
-                           cte1, (C)svisc, partials);
+        hipLaunchKernelGGL((k_dissipation<S, C, 1>), dim3(g.nparts), dim3(Tile<S>::X, Tile<S>::Y),
+                           0, s, pops_of<S>(L, cur), (const S *)gcache, g, cte1, (C)svisc,
+                           partials);
     else
-        hipLaunchKernelGGL((k_dissipation<S, C, 0>), dim3(nblocks), dim3(RED_THREADS), 0, s,
-                           pops_of<S>(L, cur), (const S *)nullptr, L.nx, L.ny, L.nz, L.zoff, pz,
-                           cte1, (C)svisc, partials);
+        hipLaunchKernelGGL((k_dissipation<S, C, 0>), dim3(g.nparts), dim3(Tile<S>::X, Tile<S>::Y),
+                           0, s, pops_of<S>(L, cur), (const S *)nullptr, g, cte1, (C)svisc,
+                           partials);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(RED_THREADS), 0, s, partials, nblocks, out);
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(FIN_THREADS), 0, s, partials, g.nparts, out);
     return hipGetLastError();
 }
 
@@ -470,13 +540,13 @@ hipError_t launch_collide(const Lattice &L, int src, const Box &box, double omeg
                           hipStream_t s) {
     BGK_DISPATCH(L.storage, collide_t, L, src, box, omega, s);
 }
-hipError_t launch_diagno(const Lattice &L, int cur, double *partials, int nblocks, double *out,
+hipError_t launch_diagno(const Lattice &L, int cur, double *partials, double *out,
                          hipStream_t s) {
-    BGK_DISPATCH(L.storage, diagno_t, L, cur, partials, nblocks, out, s);
+    BGK_DISPATCH(L.storage, diagno_t, L, cur, partials, out, s);
 }
 hipError_t launch_dissipation(const Lattice &L, int cur, const void *gcache, double svisc,
-                              double *partials, int nblocks, double *out, hipStream_t s) {
-    BGK_DISPATCH(L.storage, dissipation_t, L, cur, gcache, svisc, partials, nblocks, out, s);
+                              double *partials, double *out, hipStream_t s) {
+    BGK_DISPATCH(L.storage, dissipation_t, L, cur, gcache, svisc, partials, out, s);
 }
 hipError_t launch_pack_z(const Lattice &L, int cur, void *send, hipStream_t s) {
     BGK_DISPATCH_S(L.storage, pack_z_t, L, cur, send, s);
