@@ -1,6 +1,6 @@
 """Benchmark of the BGK3D_HIT hot path on one MI355X.
 
-Workload (one "step" = one fused stream-collide sweep of the lattice):
+Workload (one "step" = one stream-collide time step of the lattice):
 512^3 triply periodic D3Q19 BGK, seeded HIT initial field (k0 = 4,
 u_rms = 0.02, seed 1234), omega = 1.95, with the energy/dissipation
 reductions (diagno + dissipation) every 50 steps inside the timed region.
@@ -8,11 +8,17 @@ The headline `value` is the FP32 run (storage and compute FP32); the FP64
 and reduced-storage (FP16 / BF16 shifted f_i - w_i, FP32 collide) runs of
 the same workload are reported under `variants`.
 
+The time step is the two-step pass (k_collide2: two time steps per launch,
+the first step's state kept in LDS), which bgk_step takes on lattices this
+size; the one-pass-per-step kernel (k_collide) is reported beside it.
+
 `roofline` is measured live: HIP events around every collide launch on the
 stream it runs on give the average launch duration; the algorithmic bytes
-per launch are 19 loads + 19 stores per site (38 * s * 512^3).  `traffic`
-is the HBM bytes per launch measured by rocprofv3 FETCH_SIZE / WRITE_SIZE
-runs of this script (profiles/collide_traffic.json), null when absent.
+per launch are 19 loads + 19 stores of every site (38 * s * 512^3): one
+launch reads and writes the whole lattice once, for one time step
+(k_collide) or two (k_collide2).  `traffic` is the HBM bytes per launch
+measured by rocprofv3 FETCH_SIZE / WRITE_SIZE runs of this script
+(profiles/collide_traffic.json), null when absent.
 
 `cpu_baseline` times the CPU oracle (plain C/OpenMP restatement of the
 reference Fortran) on a bounded sample on the host cores.
@@ -39,12 +45,13 @@ def log(*a):
 
 
 def run_gpu(kind, n, steps, warmup, diag_every, omega=1.95, shape=None, layout="periodic",
-            k0=4.0, urms=0.02, seed=1234):
+            k0=4.0, urms=0.02, seed=1234, two_step=True):
     import bgk3d_hit_amd as bgk
 
     shape = shape or (n, n, n)
     nu = bgk.svisc_from_omega(omega)
-    lat = bgk.Lattice(*shape, omega=omega, storage=kind, layout=layout, timing=True)
+    lat = bgk.Lattice(*shape, omega=omega, storage=kind, layout=layout, timing=True,
+                      two_step=two_step)
     try:
         lat.init_hit(k0, urms, seed)
         d0 = lat.diagno()
@@ -67,14 +74,22 @@ def run_gpu(kind, n, steps, warmup, diag_every, omega=1.95, shape=None, layout="
         lat.close()
     nsite = shape[0] * shape[1] * shape[2]
     launches = max(t["collide_launches"], 1)
-    # per step: one collide launch (periodic) or bulk + 2 boundary planes (slab)
+    # per step: one collide launch (periodic), bulk + 2 boundary planes (slab),
+    # or half a launch (two-step pass)
     per_step = launches / steps
     avg_ms = t["collide_ms"] / launches
     step_collide_ms = t["collide_ms"] / steps
+    two = per_step < 0.75
+    # HBM bytes a pass must move: every population read once and written once
+    # (one launch = one step, or two steps for the two-step pass)
     algo_bytes = 38 * BYTES[kind] * nsite
-    achieved = algo_bytes / (step_collide_ms * 1e-3) / 1e9
+    pass_ms = step_collide_ms * (2 if two else 1)
+    achieved = algo_bytes / (pass_ms * 1e-3) / 1e9
     return {
         "kind": kind,
+        "two_step": two,
+        "kernel": ("k_collide2" if two else "k_collide") + f"<{kind}>",
+        "steps_per_launch": 2 if two else 1,
         "mlups": nsite * steps / wall / 1e6,
         "ms_per_step": wall * 1e3 / steps,
         "collide_avg_ms": avg_ms,
@@ -94,9 +109,10 @@ def run_gpu(kind, n, steps, warmup, diag_every, omega=1.95, shape=None, layout="
 def summary(r, ref=None):
     out = {"mlups": round(r["mlups"], 1), "ms_per_step": round(r["ms_per_step"], 4),
            "collide_ms_per_step": round(r["collide_ms_per_step"], 4),
+           "kernel": r["kernel"], "steps_per_launch": r["steps_per_launch"],
            "achieved_gbs": round(r["achieved_gbs"], 1), "frac": round(r["frac"], 4),
            "storage_bytes": BYTES[r["kind"]], "compute": COMPUTE[r["kind"]],
-           "traffic": load_traffic(r["kind"])}
+           "traffic": load_traffic(r["kind"], r["two_step"])}
     if ref is not None:
         out["speedup_vs_f32"] = round(r["mlups"] / ref["mlups"], 3)
     return out
@@ -192,12 +208,13 @@ def calibrate_copy(gib, reps=10):
                       "avg_ms": ms, "gbs": nbytes / (ms * 1e-3) / 1e9}), flush=True)
 
 
-def load_traffic(kind):
+def load_traffic(kind, two_step):
     p = os.path.join(ROOT, "profiles", "collide_traffic.json")
     try:
         with open(p) as f:
             d = json.load(f)
-        return d.get(kind, {}).get("hbm_bytes_per_launch")
+        key = ("k_collide2/" if two_step else "k_collide/") + kind
+        return d.get(key, {}).get("hbm_bytes_per_launch")
     except (OSError, ValueError):
         return None
 
@@ -232,17 +249,25 @@ def main():
         sys.exit(1)
     log(f"[bench] f32 {a.size}^3 warmup {a.warmup} steps {a.steps}")
     main_run = run_gpu("f32", a.size, a.steps, a.warmup, a.diag_every)
-    log(f"[bench] f32: {main_run['mlups']:.0f} MLUPS, collide {main_run['collide_avg_ms']:.3f} ms"
-        f" = {main_run['achieved_gbs']:.0f} GB/s")
+    log(f"[bench] f32: {main_run['mlups']:.0f} MLUPS, {main_run['kernel']} "
+        f"{main_run['collide_avg_ms']:.3f} ms/launch = {main_run['achieved_gbs']:.0f} GB/s")
     variants = {}
-    for kind in [k for k in a.variants.split(",") if k]:
-        log(f"[bench] {kind} {a.size}^3")
-        r = guarded(kind, lambda: run_gpu(kind, a.size, a.steps, a.warmup, a.diag_every))
-        if "error" in r:
-            variants[kind] = r
-            continue
-        log(f"[bench] {kind}: {r['mlups']:.0f} MLUPS, {r['achieved_gbs']:.0f} GB/s")
-        variants[kind] = summary(r, main_run)
+    single_ref = None
+    for kind in ["f32"] + [k for k in a.variants.split(",") if k and k != "f32"]:
+        for two in ((True, False) if main_run["two_step"] else (True,)):
+            if kind == "f32" and two:
+                continue
+            name = kind if two else kind + "_one_pass_per_step"
+            log(f"[bench] {name} {a.size}^3")
+            r = guarded(name, lambda: run_gpu(kind, a.size, a.steps, a.warmup, a.diag_every,
+                                              two_step=two))
+            if "error" in r:
+                variants[name] = r
+                continue
+            log(f"[bench] {name}: {r['mlups']:.0f} MLUPS, {r['kernel']} {r['achieved_gbs']:.0f} GB/s")
+            if kind == "f32":
+                single_ref = r
+            variants[name] = summary(r, main_run if two else single_ref)
     configs = {}
     if not a.no_extra:
         log("[bench] slab 1024x1024x256")
@@ -276,8 +301,10 @@ def main():
                    "layout": "SoA, halo-free, periodic wrap in index math"},
         "roofline": {"bound": "hbm", "achieved": round(main_run["achieved_gbs"], 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(main_run["frac"], 4), "traffic": load_traffic("f32"),
-                     "kernel": "k_collide<float,float>",
+                     "frac": round(main_run["frac"], 4),
+                     "traffic": load_traffic("f32", main_run["two_step"]),
+                     "kernel": main_run["kernel"],
+                     "steps_per_launch": main_run["steps_per_launch"],
                      "avg_launch_ms": round(main_run["collide_avg_ms"], 4),
                      "algo_bytes_per_launch": main_run["algo_bytes_per_launch"]},
         "cpu_baseline": cpu,

@@ -18,7 +18,8 @@ STORAGE_NAMES = {"f64": STORE_F64, "f32": STORE_F32, "f32mix": STORE_F32_MIXED,
 # layouts
 LAYOUT_PERIODIC, LAYOUT_SLAB_Z = 0, 1
 # flags
-FLAG_NOSHIFT, FLAG_REFCOMPAT_HALOS, FLAG_TIMING = 1, 2, 4
+FLAG_NOSHIFT, FLAG_REFCOMPAT_HALOS, FLAG_TIMING, FLAG_SINGLE_STEP = 1, 2, 4, 8
+FLAG_TWO_STEP_ANY_SIZE = 16
 # diagnostic modes
 DIAG_CONSISTENT, DIAG_REFCOMPAT = 0, 1
 

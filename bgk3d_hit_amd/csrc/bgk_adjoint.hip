@@ -185,9 +185,14 @@ hipError_t launch_tape_step(bgk_ctx *c, C *tape) {
     const Lattice &L = c->lat;
     int ntiles;
     Geo g = make_geo<S>(L, Box{0, L.nx, 0, L.ny, 0, L.nz}, &ntiles);
-    hipLaunchKernelGGL((k_collide<S, C, true>), dim3(ntiles), dim3(Tile<S>::X, Tile<S>::Y), 0, c->s0,
-                       pops_of<S>(L, c->cur), pops_of<S>(L, 1 - c->cur), g, (C)c->omega, (C)1,
-                       tape, L.nsite);
+    if (offsets_fit_32(L, sizeof(S)))
+        hipLaunchKernelGGL((k_collide<S, C, true, true>), dim3(ntiles),
+                           dim3(Tile<S>::X, Tile<S>::Y), 0, c->s0, pops_of<S>(L, c->cur),
+                           pops_of<S>(L, 1 - c->cur), g, (C)c->omega, (C)1, tape, L.nsite);
+    else
+        hipLaunchKernelGGL((k_collide<S, C, false, true>), dim3(ntiles),
+                           dim3(Tile<S>::X, Tile<S>::Y), 0, c->s0, pops_of<S>(L, c->cur),
+                           pops_of<S>(L, 1 - c->cur), g, (C)c->omega, (C)1, tape, L.nsite);
     c->cur ^= 1;
     return hipGetLastError();
 }

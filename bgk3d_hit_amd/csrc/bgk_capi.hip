@@ -168,6 +168,16 @@ static int record_faces(bgk_ctx *c, hipStream_t s) {
 
 int step_periodic(bgk_ctx *c, int nsteps) {
     std::vector<Box> all{full_box(c->lat)};
+    if (two_step_ok(c->lat)) {
+        for (; nsteps >= 2; nsteps -= 2) {
+            hipEvent_t ev;
+            BGK_HIP(timing_begin(c, TCLS_COLLIDE, c->s0, &ev));
+            BGK_HIP(launch_collide2(c->lat, c->cur, c->omega, c->s0));
+            BGK_HIP(timing_end(c, TCLS_COLLIDE, c->s0, ev));
+            c->cur ^= 1;
+            c->steps += 2;
+        }
+    }
     for (int t = 0; t < nsteps; t++) {
         if (c->flags & BGK_FLAG_REFCOMPAT_HALOS) {
             int r = record_faces(c, c->s0);
@@ -306,15 +316,23 @@ int bgk_create(int lx, int ly, int lz, int storage, int layout, int flags, doubl
         bgk_destroy(c);
         return code;
     };
-    hipError_t e = hipMalloc(&c->pop_alloc, pbytes * 37);
+    // Two-step fusion reads the source copy's rest population at grown-tile
+    // sites that another workgroup writes, so it needs a rest array per copy.
+    L.two_step = (layout == BGK_LAYOUT_PERIODIC &&
+                  !(flags & (BGK_FLAG_REFCOMPAT_HALOS | BGK_FLAG_SINGLE_STEP)))
+                     ? ((flags & BGK_FLAG_TWO_STEP_ANY_SIZE) ? 2 : 1)
+                     : 0;
+    const int nplanes = L.two_step ? 38 : 37;
+    hipError_t e = hipMalloc(&c->pop_alloc, pbytes * nplanes);
     if (e != hipSuccess) return bail(hip_fail(e, "hipMalloc(populations)"));
-    c->bytes += pbytes * 37;
+    c->bytes += pbytes * nplanes;
     char *base = (char *)c->pop_alloc;
     for (int q = 0; q < 18; q++) {
         L.pop[0][q] = base + pbytes * q;
         L.pop[1][q] = base + pbytes * (18 + q);
     }
-    L.pop[0][18] = L.pop[1][18] = base + pbytes * 36;
+    L.pop[0][18] = base + pbytes * 36;
+    L.pop[1][18] = base + pbytes * (nplanes - 1);
 
     if ((e = hipStreamCreateWithFlags(&c->s0, hipStreamNonBlocking)) != hipSuccess)
         return bail(hip_fail(e, "hipStreamCreate"));
@@ -324,7 +342,7 @@ int bgk_create(int lx, int ly, int lz, int storage, int layout, int flags, doubl
         return bail(hip_fail(e, "hipEventCreate"));
     if ((e = hipEventCreateWithFlags(&c->ev1, hipEventDisableTiming)) != hipSuccess)
         return bail(hip_fail(e, "hipEventCreate"));
-    if ((e = hipMemsetAsync(c->pop_alloc, 0, pbytes * 37, c->s0)) != hipSuccess)
+    if ((e = hipMemsetAsync(c->pop_alloc, 0, pbytes * nplanes, c->s0)) != hipSuccess)
         return bail(hip_fail(e, "hipMemsetAsync(populations)"));
 
     if (layout == BGK_LAYOUT_SLAB_Z) {

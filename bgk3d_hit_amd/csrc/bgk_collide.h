@@ -3,6 +3,8 @@
 // (bgk_adjoint.hip).  SRC/col_MC.F90:104-361.
 #pragma once
 
+#include <type_traits>
+
 #include "bgk_internal.h"
 #include "d3q19.h"
 
@@ -76,13 +78,37 @@ __device__ __forceinline__ int xcd_remap(int b, int G) {
     return (xcd < r) ? xcd * (q + 1) + slot : r * (q + 1) + (xcd - r) * q + slot;
 }
 
+// Element access by byte offset from a per-population base pointer.  With
+// O32 (every population array spans at most 4 GiB) the offset is a 32-bit
+// unsigned value: the base stays in scalar registers and each load or store
+// takes one vector offset register (global_load saddr form) instead of a
+// 64-bit address computed per population.
+template <bool O32>
+using off_t_ = typename std::conditional<O32, uint32_t, size_t>::type;
+
+template <typename S, typename O>
+__device__ __forceinline__ S ld_at(const S *base, O off) {
+    return *(const S *)((const char *)base + off);
+}
+template <typename S, typename O>
+__device__ __forceinline__ void st_at(S *base, O off, S v) {
+    *(S *)((char *)base + off) = v;
+}
+
+// whether byte offsets into one population array fit 32 bits
+inline bool offsets_fit_32(const Lattice &L, size_t elem_bytes) {
+    return (unsigned long long)L.plane_elems * elem_bytes <= (1ULL << 32);
+}
+
 // ------------------------------------------------------------------ collide
 // TAPE: also write the pulled-moment velocity u = j/rho of every site to
 // tape[0..2][site] (the adjoint's record of the forward pass).
-template <typename S, typename C, bool TAPE = false>
+template <typename S, typename C, bool O32, bool TAPE = false>
 __global__ __launch_bounds__(Tile<S>::X * Tile<S>::Y) void k_collide(Pops<S> src, Pops<S> dst, Geo g, C omega,
                                                  C cte1, C *__restrict__ tape = nullptr,
                                                  long long tape_ld = 0) {
+    using O = off_t_<O32>;
+    constexpr O es = sizeof(S);
     const int t = xcd_remap((int)blockIdx.x, (int)gridDim.x);
     const int tx = t % g.tiles_x;
     const int r = t / g.tiles_x;
@@ -106,29 +132,244 @@ __global__ __launch_bounds__(Tile<S>::X * Tile<S>::Y) void k_collide(Pops<S> src
         zm = zs - 1;
         zp = zs + 1;
     }
-    const int X[3] = {xm, x, xp};
-    const int Y[3] = {ym * g.nx, y * g.nx, yp * g.nx};
-    const int Z[3] = {zm * g.nxy, zs * g.nxy, zp * g.nxy};
+    // byte offsets of the x, y and z neighbours
+    const O X[3] = {(O)xm * es, (O)x * es, (O)xp * es};
+    const O Y[3] = {(O)(ym * g.nx) * es, (O)(y * g.nx) * es, (O)(yp * g.nx) * es};
+    const O Z[3] = {(O)zm * (O)g.nxy * es, (O)zs * (O)g.nxy * es, (O)zp * (O)g.nxy * es};
 
     C f[19];
 #pragma unroll
     for (int q = 0; q < 19; q++) {
         // pull: population q arrives from x - c_q
-        const int idx = X[1 - cx(q)] + Y[1 - cy(q)] + Z[1 - cz(q)];
-        f[q] = Storage<S>::template load<C>(src.p[q][idx]);
+        const O off = X[1 - cx(q)] + Y[1 - cy(q)] + Z[1 - cz(q)];
+        f[q] = Storage<S>::template load<C>(ld_at(src.p[q], off));
     }
     C b[19];
     C ux, uy, uz;
     collide_site<C>(f, b, omega, cte1, ux, uy, uz);
-    const int c = x + Y[1] + Z[1];
+    const O c = X[1] + Y[1] + Z[1];
     if (TAPE) {
-        tape[c] = ux;
-        tape[tape_ld + c] = uy;
-        tape[2 * tape_ld + c] = uz;
+        const size_t i = (size_t)(c / es);
+        tape[i] = ux;
+        tape[tape_ld + i] = uy;
+        tape[2 * tape_ld + i] = uz;
     }
 #pragma unroll
-    for (int q = 0; q < 18; q++) dst.p[q][c] = Storage<S>::template store<C>(b[q]);
-    src.p[18][c] = Storage<S>::template store<C>(b[18]);  // rest: in place
+    for (int q = 0; q < 19; q++) st_at(dst.p[q], c, Storage<S>::template store<C>(b[q]));
+}
+
+// --------------------------------------------------------- two-step collide
+// Two time steps in one pass over HBM (temporal blocking; periodic layout).
+// A workgroup owns a TX x TY column of sites over a chunk of ZC planes and
+// marches it in z.  At plane z it computes the first step S1 on the tile
+// grown by one site on every x/y side, reading the source copy S0, keeps S1
+// in LDS, and computes the second step S2 of plane z-1 from there into the
+// destination copy.  S1 goes through the storage type exactly as a store and
+// reload would, so the result equals two single steps bit for bit.
+//
+// Only the S1 values that a later neighbour pull reads are kept in LDS, each
+// for as long as it is needed (a population with c_z = d at plane p is read
+// by the second step at plane p + d):
+//   c_z = -1 with an x/y component (4): plane z, read within the same iteration
+//   c_z =  0 with an x/y component (8): plane z-1, one slot
+//   c_z = +1 with an x/y component (4): planes z-1 and z-2, two slots
+// The populations (0,0,+-1) and the rest population move only in z; they stay
+// in registers of the thread that owns the column.
+// Grown-tile shape per storage kind (X x Y interior sites, ZC planes per
+// workgroup, MINB workgroups per CU), from a sweep at 512^3.
+template <typename S>
+struct Tile2 {
+    static constexpr int X = 128, Y = 5, ZC = 64, MINB = 1;  // float, half, bfloat16
+    static constexpr bool DBUF = false;  // double-buffered loads (register cost)
+};
+template <>
+struct Tile2<double> {
+    static constexpr int X = 64, Y = 5, ZC = 64, MINB = 1;
+    static constexpr bool DBUF = false;
+};
+
+// Planes per workgroup of the two-step pass: ZC, shortened on small lattices
+// so that the launch still has about 1024 workgroups; 0 when that would take
+// chunks of fewer than 16 planes (the two extra first-step planes per chunk
+// would cost more than the pass saves: take single steps).
+constexpr int TWO_STEP_MIN_BLOCKS = 1024, TWO_STEP_MIN_ZC = 16;
+// L.two_step == 2 takes chunks of at least min(16, nz) planes at any size.
+inline int two_step_zc(const Lattice &L, int tx, int ty, int zc_max) {
+    const long long txy = (long long)((L.nx + tx - 1) / tx) * ((L.ny + ty - 1) / ty);
+    long long zc = txy * L.nz / TWO_STEP_MIN_BLOCKS;
+    if (L.two_step == 2 && zc < TWO_STEP_MIN_ZC) zc = TWO_STEP_MIN_ZC;
+    if (zc > zc_max) zc = zc_max;
+    if (zc > L.nz) zc = L.nz;
+    return zc < TWO_STEP_MIN_ZC && zc < L.nz ? 0 : (int)zc;
+}
+
+// population group: 0 c_z = -1, 1 c_z = 0, 2 c_z = +1 (all with an x/y
+// component), 3 no x/y component
+__host__ __device__ constexpr int pgroup(int q) {
+    return (cx(q) == 0 && cy(q) == 0) ? 3 : (cz(q) < 0 ? 0 : (cz(q) == 0 ? 1 : 2));
+}
+// index of q within its group
+__host__ __device__ constexpr int pslot(int q) {
+    int k = 0;
+    for (int i = 0; i < q; i++) k += (pgroup(i) == pgroup(q)) ? 1 : 0;
+    return k;
+}
+// first LDS plane of a group (group 2: plus 4 per slot)
+__host__ __device__ constexpr int pbase(int grp) { return grp == 0 ? 0 : (grp == 1 ? 4 : 12); }
+
+struct Geo2 {
+    int nx, ny, nz, nxy;
+    int tiles_x, tiles_y, zc;
+};
+
+static Geo2 make_geo2(const Lattice &L, int tx, int ty, int zc, int *nblocks) {
+    Geo2 g;
+    g.nx = L.nx;
+    g.ny = L.ny;
+    g.nz = L.nz;
+    g.nxy = L.nx * L.ny;
+    g.tiles_x = (L.nx + tx - 1) / tx;
+    g.tiles_y = (L.ny + ty - 1) / ty;
+    g.zc = zc;
+    *nblocks = g.tiles_x * g.tiles_y * ((L.nz + g.zc - 1) / g.zc);
+    return g;
+}
+
+// threads: one per site of the TX-wide x range of the TY+2 grown rows, plus
+// one wave for the two grown columns
+constexpr int collide2_threads(int tx, int ty) { return tx * (ty + 2) + 64; }
+
+template <typename S, typename C, bool O32, int TX, int TY, int MINB, bool DBUF>
+__global__ __launch_bounds__(collide2_threads(TX, TY), MINB) void k_collide2(Pops<S> src,
+                                                                             Pops<S> dst, Geo2 g,
+                                                                             C omega, C cte1) {
+    using O = off_t_<O32>;
+    constexpr O es = sizeof(S);
+    constexpr int EX = TX + 2, EY = TY + 2;
+    constexpr int NMAIN = TX * EY;  // threads of the x range [0, TX) of every row
+    __shared__ S lds[20][EY][EX];
+
+    const int t = xcd_remap((int)blockIdx.x, (int)gridDim.x);
+    const int tx = t % g.tiles_x;
+    const int r = t / g.tiles_x;
+    const int ty = r % g.tiles_y;
+    const int za = (r / g.tiles_y) * g.zc;
+    const int zb = min(g.nz, za + g.zc);
+
+    // site of the grown tile handled by this thread: (ex, ey) in [0,EX) x [0,EY);
+    // the last wave takes the columns ex = 0 and ex = EX-1
+    const int tid = (int)threadIdx.x;
+    int ex, ey;
+    bool active;
+    if (tid < NMAIN) {
+        ex = tid % TX + 1;
+        ey = tid / TX;
+        active = true;
+    } else {
+        const int e = tid - NMAIN;
+        active = e < 2 * EY;
+        ex = (e < EY) ? 0 : EX - 1;
+        ey = e % EY;
+    }
+    const int lx = tx * TX + ex - 1, ly = ty * TY + ey - 1;  // may leave the box
+    const int gx = ((lx % g.nx) + g.nx) % g.nx;
+    const int gy = ((ly % g.ny) + g.ny) % g.ny;
+    // the thread that computes S1 of an interior column also computes its S2
+    const bool owner = tid < NMAIN && ey >= 1 && ey <= TY && lx < g.nx && ly < g.ny;
+
+    const int xm = (gx == 0) ? g.nx - 1 : gx - 1;
+    const int xp = (gx == g.nx - 1) ? 0 : gx + 1;
+    const int ym = (gy == 0) ? g.ny - 1 : gy - 1;
+    const int yp = (gy == g.ny - 1) ? 0 : gy + 1;
+    // byte offsets of the x and y neighbours
+    const O X[3] = {(O)xm * es, (O)gx * es, (O)xp * es};
+    const O Y[3] = {(O)(ym * g.nx) * es, (O)(gy * g.nx) * es, (O)(yp * g.nx) * es};
+    const O plane = (O)g.nxy * es;
+
+    auto wrapz = [&](int z) { return z < 0 ? z + g.nz : (z >= g.nz ? z - g.nz : z); };
+    auto load_plane = [&](int z, S(&raw)[19]) {  // z in [-1, nz]
+        const int p = wrapz(z);
+        const int zm = (p == 0) ? g.nz - 1 : p - 1;
+        const int zp = (p == g.nz - 1) ? 0 : p + 1;
+        const O Z[3] = {(O)zm * plane, (O)p * plane, (O)zp * plane};
+#pragma unroll
+        for (int q = 0; q < 19; q++)
+            raw[q] = ld_at(src.p[q], X[1 - cx(q)] + Y[1 - cy(q)] + Z[1 - cz(q)]);
+    };
+
+    S r5_1 = S{}, r5_2 = S{}, r18_1 = S{};  // S1 of this column at planes z-1, z-2
+    const int nit = zb - za + 2;            // iterations z = za-1 .. zb
+    // iteration `it`: S1 of plane z from cur (loaded one iteration earlier),
+    // loads of plane z+1 into nxt, S2 of plane z-1
+    auto iteration = [&](int it, S(&cur)[19], S(&nxt)[19]) {
+        const int z = za - 1 + it;
+        const int pb = pbase(2) + 4 * (it & 1);  // slot of S1(z-2), then of S1(z)
+        S s1[19];
+        if (active) {
+            C f[19];
+#pragma unroll
+            for (int q = 0; q < 19; q++) f[q] = Storage<S>::template load<C>(cur[q]);
+            if (it + 1 < nit) load_plane(z + 1, nxt);  // in flight during this iteration
+            C b[19];
+            C ux, uy, uz;
+            collide_site<C>(f, b, omega, cte1, ux, uy, uz);
+#pragma unroll
+            for (int q = 0; q < 19; q++) s1[q] = Storage<S>::template store<C>(b[q]);
+#pragma unroll
+            for (int q = 0; q < 19; q++)
+                if (pgroup(q) == 0) lds[pbase(0) + pslot(q)][ey][ex] = s1[q];
+        }
+        __syncthreads();
+        if (owner && it >= 2) {  // second step at plane z-1
+            C f[19];
+#pragma unroll
+            for (int q = 0; q < 19; q++) {
+                S v;
+                if (pgroup(q) == 0)
+                    v = lds[pbase(0) + pslot(q)][ey - cy(q)][ex - cx(q)];  // S1(z)
+                else if (pgroup(q) == 1)
+                    v = lds[pbase(1) + pslot(q)][ey - cy(q)][ex - cx(q)];  // S1(z-1)
+                else if (pgroup(q) == 2)
+                    v = lds[pb + pslot(q)][ey - cy(q)][ex - cx(q)];  // S1(z-2)
+                else if (cz(q) > 0)
+                    v = r5_2;
+                else if (cz(q) < 0)
+                    v = s1[q];
+                else
+                    v = r18_1;
+                f[q] = Storage<S>::template load<C>(v);
+            }
+            C b[19];
+            C ux, uy, uz;
+            collide_site<C>(f, b, omega, cte1, ux, uy, uz);
+            const O c = X[1] + Y[1] + (O)wrapz(z - 1) * plane;
+#pragma unroll
+            for (int q = 0; q < 19; q++) st_at(dst.p[q], c, Storage<S>::template store<C>(b[q]));
+        }
+        __syncthreads();
+        if (active) {
+#pragma unroll
+            for (int q = 0; q < 19; q++) {
+                if (pgroup(q) == 1) lds[pbase(1) + pslot(q)][ey][ex] = s1[q];
+                if (pgroup(q) == 2) lds[pb + pslot(q)][ey][ex] = s1[q];
+            }
+        }
+        r5_2 = r5_1;
+        r5_1 = s1[5];
+        r18_1 = s1[18];
+    };
+
+    S ra[19];
+    if (active) load_plane(za - 1, ra);
+    if (DBUF) {  // two register sets for the loads in flight, alternating
+        S rb[19];
+        for (int it = 0; it < nit; it += 2) {
+            iteration(it, ra, rb);
+            if (it + 1 < nit) iteration(it + 1, rb, ra);
+        }
+    } else {  // one set: the values are copied out before the next loads
+        for (int it = 0; it < nit; it++) iteration(it, ra, ra);
+    }
 }
 
 }  // namespace bgk

@@ -31,8 +31,11 @@ struct Lattice {
     int shifted;
     long long nsite;       // nx*ny*nz
     long long plane_elems; // nx*ny*nzs stored values per population
-    void *pop[2][19];      // population base pointers of copy 0/1; [c][18] is the
-                           // shared in-place rest population
+    void *pop[2][19];      // population base pointers of copy 0/1; the rest
+                           // population [c][18] is one shared array (updated in
+                           // place) unless two_step, then one per copy
+    int two_step;          // bgk_step may fuse two time steps per pass: 0 no,
+                           // 1 where it pays, 2 at any size
 };
 
 struct Timing {
@@ -64,7 +67,7 @@ struct bgk_ctx {
     hipStream_t s1 = nullptr;  // halo / boundary stream (slab overlap)
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
 
-    void *pop_alloc = nullptr;  // one allocation holding 37 population planes
+    void *pop_alloc = nullptr;  // one allocation holding 37 (38) population planes
 
     // slab halo message: [face 0 = z+ (plane nz), 1 = z- (plane 1)][5][nx*ny]
     void *halo_send = nullptr;
@@ -110,6 +113,10 @@ hipError_t timing_end(bgk_ctx *c, int cls, hipStream_t s, hipEvent_t begin);
 // fused pull-stream + collide from copy `src` to copy 1-src over `box`
 hipError_t launch_collide(const Lattice &L, int src, const Box &box, double omega,
                           hipStream_t s);
+// two time steps in one pass from copy `src` into copy 1-src (periodic layout
+// with a rest array per copy); only when two_step_ok(L)
+bool two_step_ok(const Lattice &L);
+hipError_t launch_collide2(const Lattice &L, int src, double omega, hipStream_t s);
 // reductions: at-site (diagno) and pulled (dissipation)
 //   mode 0: pulled gather with periodic wrap (or slab ghosts as stored)
 //   mode 1: pulled gather reading face images from gcache (periodic layout)

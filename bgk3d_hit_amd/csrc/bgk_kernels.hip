@@ -27,8 +27,34 @@ static hipError_t collide_t(const Lattice &L, int src, const Box &box, double om
     int ntiles;
     Geo g = make_geo<S>(L, box, &ntiles);
     Pops<S> a = pops_of<S>(L, src), b = pops_of<S>(L, 1 - src);
-    hipLaunchKernelGGL((k_collide<S, C>), dim3(ntiles), dim3(Tile<S>::X, Tile<S>::Y), 0, s, a, b,
-                       g, (C)omega, (C)(L.shifted ? 1.0 : 0.0));
+    const C w = (C)omega, cte1 = (C)(L.shifted ? 1.0 : 0.0);
+    if (offsets_fit_32(L, sizeof(S)))
+        hipLaunchKernelGGL((k_collide<S, C, true>), dim3(ntiles), dim3(Tile<S>::X, Tile<S>::Y), 0,
+                           s, a, b, g, w, cte1);
+    else
+        hipLaunchKernelGGL((k_collide<S, C, false>), dim3(ntiles), dim3(Tile<S>::X, Tile<S>::Y), 0,
+                           s, a, b, g, w, cte1);
+    return hipGetLastError();
+}
+
+// two steps from copy `src` into copy 1-src (periodic layout, separate rest
+// arrays per copy)
+template <typename S, typename C>
+static hipError_t collide2_t(const Lattice &L, int src, double omega, hipStream_t s) {
+    using T = Tile2<S>;
+    const int zc = two_step_zc(L, T::X, T::Y, T::ZC);
+    if (zc <= 0) return hipErrorInvalidValue;  // caller checks two_step_ok first
+    int nblocks;
+    Geo2 g = make_geo2(L, T::X, T::Y, zc, &nblocks);
+    Pops<S> a = pops_of<S>(L, src), b = pops_of<S>(L, 1 - src);
+    const C w = (C)omega, cte1 = (C)(L.shifted ? 1.0 : 0.0);
+    const dim3 block(collide2_threads(T::X, T::Y));
+    if (offsets_fit_32(L, sizeof(S)))
+        hipLaunchKernelGGL((k_collide2<S, C, true, T::X, T::Y, T::MINB, T::DBUF>), dim3(nblocks), block, 0,
+                           s, a, b, g, w, cte1);
+    else
+        hipLaunchKernelGGL((k_collide2<S, C, false, T::X, T::Y, T::MINB, T::DBUF>), dim3(nblocks), block,
+                           0, s, a, b, g, w, cte1);
     return hipGetLastError();
 }
 
@@ -539,6 +565,23 @@ static hipError_t gather_box_t(const Lattice &L, int cur, const Box &b, double *
 hipError_t launch_collide(const Lattice &L, int src, const Box &box, double omega,
                           hipStream_t s) {
     BGK_DISPATCH(L.storage, collide_t, L, src, box, omega, s);
+}
+template <typename S, typename C>
+static bool two_step_ok_t(const Lattice &L) {
+    return L.two_step && two_step_zc(L, Tile2<S>::X, Tile2<S>::Y, Tile2<S>::ZC) > 0;
+}
+bool two_step_ok(const Lattice &L) {
+    switch (L.storage) {
+    case BGK_STORE_F64: return two_step_ok_t<double, double>(L);
+    case BGK_STORE_F32:
+    case BGK_STORE_F32_MIXED: return two_step_ok_t<float, float>(L);
+    case BGK_STORE_F16: return two_step_ok_t<half_t, float>(L);
+    case BGK_STORE_BF16: return two_step_ok_t<bf16_t, float>(L);
+    default: return false;
+    }
+}
+hipError_t launch_collide2(const Lattice &L, int src, double omega, hipStream_t s) {
+    BGK_DISPATCH(L.storage, collide2_t, L, src, omega, s);
 }
 hipError_t launch_diagno(const Lattice &L, int cur, double *partials, double *out,
                          hipStream_t s) {

@@ -56,6 +56,10 @@ typedef struct bgk_ctx bgk_ctx;
 #define BGK_FLAG_REFCOMPAT_HALOS 2 /* keep the stale-halo images the reference's
                                       dissipation reads (periodic layout)          */
 #define BGK_FLAG_TIMING 4          /* record HIP events around every kernel launch  */
+#define BGK_FLAG_SINGLE_STEP 8     /* bgk_step: one pass over the populations per
+                                      time step (no two-step fusion)               */
+#define BGK_FLAG_TWO_STEP_ANY_SIZE 16 /* bgk_step: fuse two steps per pass also on
+                                      lattices too small for it to pay (tests)     */
 
 /* diagnostic modes */
 #define BGK_DIAG_CONSISTENT 0 /* gathers read the current state (fresh periodic images) */

@@ -209,14 +209,52 @@ def test_refcompat_dissipation_needs_flag(gpu):
     assert e.value.code == 4
 
 
-def test_timing_counters(gpu):
-    g = bgk.Lattice(64, 64, 64, omega=1.9, storage="f32", timing=True)
+@pytest.mark.parametrize("two_step,launches", [("always", 5), (True, 10), (False, 10)])
+def test_timing_counters(gpu, two_step, launches):
+    g = bgk.Lattice(64, 64, 64, omega=1.9, storage="f32", timing=True, two_step=two_step)
     g.init_hit(4.0, 0.02, 1)
     g.step(10)
     g.diagno()
     t = g.timing()
-    assert t["collide_launches"] == 10 and t["reduce_launches"] == 1
+    assert t["collide_launches"] == launches and t["reduce_launches"] == 1
     assert t["collide_ms"] > 0 and t["step_ms"] >= t["collide_ms"] * 0.5
+
+
+@pytest.mark.parametrize("kind", ["f64", "f32", "f32mix", "f16", "bf16"])
+@pytest.mark.parametrize("shape", [(70, 9, 6), (64, 16, 130), (24, 20, 16), (5, 3, 1),
+                                   (130, 17, 65), (1, 1, 2)])
+def test_two_step_pass_equals_single_steps(gpu, kind, shape):
+    """bgk_step fuses two time steps per pass over the populations (grown
+    tiles, first step kept in LDS); it must give the single-step result bit
+    for bit, at sizes that leave partial tiles and a partial last z chunk,
+    and for odd step counts (a fused pass followed by one single step)."""
+    say("two-step vs single", kind, shape)
+    u = bgk.hit_velocity(*shape, 2.0, 0.03, 11)
+    a = bgk.Lattice(*shape, omega=1.93, storage=kind, two_step="always")
+    b = bgk.Lattice(*shape, omega=1.93, storage=kind, two_step=False)
+    a.init_equilibrium(u)
+    b.init_equilibrium(u)
+    for n in (2, 3, 1, 4):
+        a.step(n)
+        for _ in range(n):
+            b.col_MC()
+        assert np.array_equal(a.get_populations(), b.get_populations()), n
+
+
+@pytest.mark.parametrize("kind", ["f32", "f16", "f64"])
+def test_two_step_pass_256cubed_checksum(gpu, kind):
+    """At 256^3 step() takes the two-step pass by itself (10 launches for 20
+    steps) and matches single steps bit for bit."""
+    say("two-step 256^3", kind)
+    u = bgk.hit_velocity(256, 256, 256, 4.0, 0.02, 5)
+    a = bgk.Lattice(256, 256, 256, omega=1.95, storage=kind, timing=True)
+    b = bgk.Lattice(256, 256, 256, omega=1.95, storage=kind, two_step=False)
+    a.init_equilibrium(u)
+    b.init_equilibrium(u)
+    a.step(20)
+    b.step(20)
+    assert a.timing()["collide_launches"] == 10
+    assert a.checksum() == b.checksum()
 
 
 def test_512_cubed_fp32_against_fp64(gpu):
