@@ -245,13 +245,15 @@ int adjoint_t(bgk_ctx *c, const double *u0, int T, int K, int levels, double *J,
     const int last_s0 = nseg > 0 ? (nseg - 1) * K : 0;
     auto t0 = std::chrono::steady_clock::now();
     BGK_HIP(launch_init_equilibrium(L, c->cur, u0, N, 0, N, c->s0));
-    for (int t = 0; t < T; t++) {
+    for (int t = 0; t < T;) {
         if (t % K2 == 0 && (r = a.save(coarse[t / K2]))) return r;
         if (t >= last_s0) {
             BGK_HIP((launch_tape_step<S, C>(c, tape + (size_t)(t - last_s0) * 3 * N)));
-        } else {
-            BGK_HIP(launch_collide(L, c->cur, Box{0, L.nx, 0, L.ny, 0, L.nz}, c->omega, c->s0));
-            c->cur ^= 1;
+            t++;
+        } else {  // untaped steps up to the next checkpoint (two-step passes where they pay)
+            const int n = std::min(K2 - t % K2, last_s0 - t);
+            if ((r = step_periodic(c, n, false))) return r;
+            t += n;
         }
     }
     if ((r = reduce_ke(c, c->cur, J))) return r;  // synchronises
@@ -276,12 +278,8 @@ int adjoint_t(bgk_ctx *c, const double *u0, int T, int K, int levels, double *J,
                 for (int j = 0; j < need; j++) {
                     if ((r = a.save(fine[j]))) return r;
                     if (j == need - 1) break;
-                    for (int t = 0; t < K; t++) {
-                        BGK_HIP(launch_collide(L, c->cur, Box{0, L.nx, 0, L.ny, 0, L.nz},
-                                               c->omega, c->s0));
-                        c->cur ^= 1;
-                        a.recomputed++;
-                    }
+                    if ((r = step_periodic(c, K, false))) return r;
+                    a.recomputed += K;
                 }
             }
         }
@@ -324,11 +322,8 @@ template <typename S, typename C>
 int forward_t(bgk_ctx *c, const double *u0, int T, double *J) {
     const Lattice &L = c->lat;
     BGK_HIP(launch_init_equilibrium(L, c->cur, u0, L.nsite, 0, L.nsite, c->s0));
-    for (int t = 0; t < T; t++) {
-        BGK_HIP(launch_collide(L, c->cur, Box{0, L.nx, 0, L.ny, 0, L.nz}, c->omega, c->s0));
-        c->cur ^= 1;
-    }
-    return reduce_ke(c, c->cur, J);
+    int r = step_periodic(c, T, false);
+    return r ? r : reduce_ke(c, c->cur, J);
 }
 
 }  // namespace

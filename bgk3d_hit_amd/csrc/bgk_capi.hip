@@ -166,8 +166,9 @@ static int record_faces(bgk_ctx *c, hipStream_t s) {
     return BGK_OK;
 }
 
-int step_periodic(bgk_ctx *c, int nsteps) {
+int step_periodic(bgk_ctx *c, int nsteps, bool count) {
     std::vector<Box> all{full_box(c->lat)};
+    const int n0 = nsteps;
     if (two_step_ok(c->lat)) {
         for (; nsteps >= 2; nsteps -= 2) {
             hipEvent_t ev;
@@ -175,7 +176,6 @@ int step_periodic(bgk_ctx *c, int nsteps) {
             BGK_HIP(launch_collide2(c->lat, c->cur, c->omega, c->s0));
             BGK_HIP(timing_end(c, TCLS_COLLIDE, c->s0, ev));
             c->cur ^= 1;
-            c->steps += 2;
         }
     }
     for (int t = 0; t < nsteps; t++) {
@@ -186,8 +186,8 @@ int step_periodic(bgk_ctx *c, int nsteps) {
         int r = collide_boxes(c, all, c->s0);
         if (r) return r;
         c->cur ^= 1;
-        c->steps++;
     }
+    if (count) c->steps += n0;
     return BGK_OK;
 }
 

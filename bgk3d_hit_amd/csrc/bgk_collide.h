@@ -175,17 +175,15 @@ __global__ __launch_bounds__(Tile<S>::X * Tile<S>::Y) void k_collide(Pops<S> src
 //   c_z = +1 with an x/y component (4): planes z-1 and z-2, two slots
 // The populations (0,0,+-1) and the rest population move only in z; they stay
 // in registers of the thread that owns the column.
-// Grown-tile shape per storage kind (X x Y interior sites, ZC planes per
-// workgroup, MINB workgroups per CU), from a sweep at 512^3.
-template <typename S>
+// Grown-tile shape (X x Y interior sites, ZC planes per workgroup, MINB
+// workgroups per CU), from a sweep at 512^3: 128 x 5 (960 threads) for FP32
+// compute with 32-bit offsets; 64 x 5 (512 threads, twice the registers per
+// thread) for FP64 compute or 64-bit offsets.
+template <typename S, typename C, bool O32>
 struct Tile2 {
-    static constexpr int X = 128, Y = 5, ZC = 64, MINB = 1;  // float, half, bfloat16
+    static constexpr bool wide = sizeof(C) == 4 && O32;
+    static constexpr int X = wide ? 128 : 64, Y = 5, ZC = 64, MINB = 1;
     static constexpr bool DBUF = false;  // double-buffered loads (register cost)
-};
-template <>
-struct Tile2<double> {
-    static constexpr int X = 64, Y = 5, ZC = 64, MINB = 1;
-    static constexpr bool DBUF = false;
 };
 
 // Planes per workgroup of the two-step pass: ZC, shortened on small lattices
@@ -238,6 +236,17 @@ static Geo2 make_geo2(const Lattice &L, int tx, int ty, int zc, int *nblocks) {
 // threads: one per site of the TX-wide x range of the TY+2 grown rows, plus
 // one wave for the two grown columns
 constexpr int collide2_threads(int tx, int ty) { return tx * (ty + 2) + 64; }
+
+// Workgroup barrier that orders LDS accesses only.  __syncthreads() also
+// waits for every global load in flight, which would drain the next plane's
+// loads issued before it; the two-step pass shares data between threads
+// through LDS alone (it reads only the source copy and writes only the
+// destination copy).
+__device__ __forceinline__ void lds_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
 
 template <typename S, typename C, bool O32, int TX, int TY, int MINB, bool DBUF>
 __global__ __launch_bounds__(collide2_threads(TX, TY), MINB) void k_collide2(Pops<S> src,
@@ -305,11 +314,19 @@ __global__ __launch_bounds__(collide2_threads(TX, TY), MINB) void k_collide2(Pop
         const int z = za - 1 + it;
         const int pb = pbase(2) + 4 * (it & 1);  // slot of S1(z-2), then of S1(z)
         S s1[19];
-        if (active) {
-            C f[19];
+        // every thread loads (idle lanes of the last wave at valid addresses),
+        // so the registers in flight need no merge at the end of `if (active)`
+        C f[19];
 #pragma unroll
-            for (int q = 0; q < 19; q++) f[q] = Storage<S>::template load<C>(cur[q]);
-            if (it + 1 < nit) load_plane(z + 1, nxt);  // in flight during this iteration
+        for (int q = 0; q < 19; q++) {
+            f[q] = Storage<S>::template load<C>(cur[q]);
+            // a copy of its own: the next loads can then land in cur's
+            // registers, which carry them around the loop without a copy
+            // (a copy at the loop end would wait for the loads)
+            asm volatile("" : "+v"(f[q]));
+        }
+        if (it + 1 < nit) load_plane(z + 1, nxt);  // in flight during this iteration
+        if (active) {
             C b[19];
             C ux, uy, uz;
             collide_site<C>(f, b, omega, cte1, ux, uy, uz);
@@ -319,7 +336,7 @@ __global__ __launch_bounds__(collide2_threads(TX, TY), MINB) void k_collide2(Pop
             for (int q = 0; q < 19; q++)
                 if (pgroup(q) == 0) lds[pbase(0) + pslot(q)][ey][ex] = s1[q];
         }
-        __syncthreads();
+        lds_barrier();
         if (owner && it >= 2) {  // second step at plane z-1
             C f[19];
 #pragma unroll
@@ -346,7 +363,7 @@ __global__ __launch_bounds__(collide2_threads(TX, TY), MINB) void k_collide2(Pop
 #pragma unroll
             for (int q = 0; q < 19; q++) st_at(dst.p[q], c, Storage<S>::template store<C>(b[q]));
         }
-        __syncthreads();
+        lds_barrier();
         if (active) {
 #pragma unroll
             for (int q = 0; q < 19; q++) {
@@ -360,7 +377,7 @@ __global__ __launch_bounds__(collide2_threads(TX, TY), MINB) void k_collide2(Pop
     };
 
     S ra[19];
-    if (active) load_plane(za - 1, ra);
+    load_plane(za - 1, ra);
     if (DBUF) {  // two register sets for the loads in flight, alternating
         S rb[19];
         for (int it = 0; it < nit; it += 2) {

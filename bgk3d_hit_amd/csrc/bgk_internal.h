@@ -168,6 +168,6 @@ int forward_ke(bgk_ctx *c, const double *u0_dev, int nsteps, double *J);
 
 // shared helpers used by the adjoint
 int reduce_ke(bgk_ctx *c, int copy, double *J);
-int step_periodic(bgk_ctx *c, int nsteps);
+int step_periodic(bgk_ctx *c, int nsteps, bool count = true);
 
 }  // namespace bgk

@@ -39,23 +39,24 @@ static hipError_t collide_t(const Lattice &L, int src, const Box &box, double om
 
 // two steps from copy `src` into copy 1-src (periodic layout, separate rest
 // arrays per copy)
-template <typename S, typename C>
-static hipError_t collide2_t(const Lattice &L, int src, double omega, hipStream_t s) {
-    using T = Tile2<S>;
+template <typename S, typename C, bool O32>
+static hipError_t collide2_o(const Lattice &L, int src, double omega, hipStream_t s) {
+    using T = Tile2<S, C, O32>;
     const int zc = two_step_zc(L, T::X, T::Y, T::ZC);
     if (zc <= 0) return hipErrorInvalidValue;  // caller checks two_step_ok first
     int nblocks;
     Geo2 g = make_geo2(L, T::X, T::Y, zc, &nblocks);
-    Pops<S> a = pops_of<S>(L, src), b = pops_of<S>(L, 1 - src);
-    const C w = (C)omega, cte1 = (C)(L.shifted ? 1.0 : 0.0);
-    const dim3 block(collide2_threads(T::X, T::Y));
-    if (offsets_fit_32(L, sizeof(S)))
-        hipLaunchKernelGGL((k_collide2<S, C, true, T::X, T::Y, T::MINB, T::DBUF>), dim3(nblocks), block, 0,
-                           s, a, b, g, w, cte1);
-    else
-        hipLaunchKernelGGL((k_collide2<S, C, false, T::X, T::Y, T::MINB, T::DBUF>), dim3(nblocks), block,
-                           0, s, a, b, g, w, cte1);
+    hipLaunchKernelGGL((k_collide2<S, C, O32, T::X, T::Y, T::MINB, T::DBUF>), dim3(nblocks),
+                       dim3(collide2_threads(T::X, T::Y)), 0, s, pops_of<S>(L, src),
+                       pops_of<S>(L, 1 - src), g, (C)omega, (C)(L.shifted ? 1.0 : 0.0));
     return hipGetLastError();
+}
+// two steps from copy `src` into copy 1-src (periodic layout, separate rest
+// arrays per copy)
+template <typename S, typename C>
+static hipError_t collide2_t(const Lattice &L, int src, double omega, hipStream_t s) {
+    return offsets_fit_32(L, sizeof(S)) ? collide2_o<S, C, true>(L, src, omega, s)
+                                        : collide2_o<S, C, false>(L, src, omega, s);
 }
 
 // --------------------------------------------------------------- reductions
@@ -568,13 +569,18 @@ hipError_t launch_collide(const Lattice &L, int src, const Box &box, double omeg
 }
 template <typename S, typename C>
 static bool two_step_ok_t(const Lattice &L) {
-    return L.two_step && two_step_zc(L, Tile2<S>::X, Tile2<S>::Y, Tile2<S>::ZC) > 0;
+    if (!L.two_step) return false;
+    return offsets_fit_32(L, sizeof(S))
+               ? two_step_zc(L, Tile2<S, C, true>::X, Tile2<S, C, true>::Y,
+                             Tile2<S, C, true>::ZC) > 0
+               : two_step_zc(L, Tile2<S, C, false>::X, Tile2<S, C, false>::Y,
+                             Tile2<S, C, false>::ZC) > 0;
 }
 bool two_step_ok(const Lattice &L) {
     switch (L.storage) {
     case BGK_STORE_F64: return two_step_ok_t<double, double>(L);
-    case BGK_STORE_F32:
-    case BGK_STORE_F32_MIXED: return two_step_ok_t<float, float>(L);
+    case BGK_STORE_F32: return two_step_ok_t<float, float>(L);
+    case BGK_STORE_F32_MIXED: return two_step_ok_t<float, double>(L);
     case BGK_STORE_F16: return two_step_ok_t<half_t, float>(L);
     case BGK_STORE_BF16: return two_step_ok_t<bf16_t, float>(L);
     default: return false;
