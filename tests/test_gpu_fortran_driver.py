@@ -129,3 +129,29 @@ def test_driver_probe_vtk_and_restart(gpu, tmp_path):
     exp = (j.astype(np.float32) / rho.astype(np.float32)).reshape(3, -1).T
     assert np.allclose(data, exp, rtol=2e-6, atol=1e-9)
     assert (a / "tec_xz.00000.00000040.vtk").exists() and (a / "tec_yz.00000.00000000.vtk").exists()
+
+
+@pytest.mark.gpu
+def test_driver_two_step_pairs_match_single_steps(gpu, tmp_path):
+    """Between output steps the driver hands the library two steps per call
+    (bgk_step(2): one two-step pass on a 256^3 lattice); the diagnostics at
+    every icheck equal those of single steps (bgk_col_MC) exactly."""
+    n, omega, itfin, icheck = 256, 1.95, 21, 10
+    svisc = (2.0 / omega - 1.0) / 6.0  # the driver's hencol with omega given
+    deck = (f"&parameters\nlx={n}\nly={n}\nlz={n}\nomega={omega}\nitfin={itfin}\n"
+            f"icheck={icheck}\ninit='hit'\nk0=4.0\nurms=0.02\nseed=1234\n"
+            f"storage='f32'\ndiag='consistent'\n/\n")
+    print("[gpu-case] fortran driver 256^3 f32 two-step pairs", flush=True)
+    run_driver(tmp_path, deck)
+    dg = np.loadtxt(tmp_path / "diagno.full.dat", ndmin=2)
+    ds = np.loadtxt(tmp_path / "dissipation.full.dat", ndmin=2)
+    lat = bgk.Lattice(n, n, n, omega=omega, storage="f32", two_step=False)
+    lat.init_hit(4.0, 0.02, 1234)
+    for row, it in enumerate(range(icheck, itfin + 1, icheck)):
+        lat.step(icheck)
+        d = lat.diagno()
+        s = lat.dissipation(svisc)
+        assert dg[row, 0] == it and ds[row, 0] == it
+        assert np.array_equal(dg[row, 1:], [d[1], d[2], d[3], d[0], d[4]]), (it, dg[row], d)
+        assert ds[row, 2] == s[1] and ds[row, 3] == s[0], (it, ds[row], s)
+    assert dg.shape[0] == itfin // icheck
