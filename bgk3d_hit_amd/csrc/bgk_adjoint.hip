@@ -61,6 +61,30 @@ __global__ __launch_bounds__(256) void k_lambda_final(Pops<S> P, C *M, int nx, i
     }
 }
 
+// mu_k = (1 - omega) lambda_k + omega [A + (B - 3 S u) . c_k] at one site
+template <typename C>
+__device__ __forceinline__ void adjoint_site(const C (&lam)[19], C ux, C uy, C uz, C omega,
+                                             C (&mu)[19]) {
+    const C usq = ux * ux + uy * uy + uz * uz;
+    C A = 0, Bx = 0, By = 0, Bz = 0, Sw = 0;
+#pragma unroll
+    for (int q = 0; q < 19; q++) {
+        const C wl = (C)wq(q) * lam[q];
+        const C cu = (C)cx(q) * ux + (C)cy(q) * uy + (C)cz(q) * uz;
+        A += wl * ((C)1 - (C)4.5 * cu * cu + (C)1.5 * usq);
+        const C s = wl * ((C)3 + (C)9 * cu);
+        Bx += s * (C)cx(q);
+        By += s * (C)cy(q);
+        Bz += s * (C)cz(q);
+        Sw += wl;
+    }
+    const C Gx = Bx - (C)3 * Sw * ux, Gy = By - (C)3 * Sw * uy, Gz = Bz - (C)3 * Sw * uz;
+#pragma unroll
+    for (int q = 0; q < 19; q++)
+        mu[q] = ((C)1 - omega) * lam[q] +
+                omega * (A + Gx * (C)cx(q) + Gy * (C)cy(q) + Gz * (C)cz(q));
+}
+
 // one adjoint step: lambda^{t+1}_i(x) = M^{t+1}_i(x + c_i) (shifted loads),
 // tape u^t(x); writes M^t_k(x) = mu_k(x)
 template <typename C>
@@ -89,25 +113,47 @@ __global__ __launch_bounds__(256) void k_adjoint_step(const C *__restrict__ Min,
     for (int q = 0; q < 19; q++)
         lam[q] = Min[(size_t)q * N + X[1 + cx(q)] + Y[1 + cy(q)] + Z[1 + cz(q)]];
     const int c = x + Y[1] + Z[1];
-    const C ux = tape[c], uy = tape[N + c], uz = tape[2 * N + c];
-    const C usq = ux * ux + uy * uy + uz * uz;
-    C A = 0, Bx = 0, By = 0, Bz = 0, Sw = 0;
+    C mu[19];
+    adjoint_site<C>(lam, tape[c], tape[N + c], tape[2 * N + c], omega, mu);
 #pragma unroll
-    for (int q = 0; q < 19; q++) {
-        const C wl = (C)wq(q) * lam[q];
-        const C cu = (C)cx(q) * ux + (C)cy(q) * uy + (C)cz(q) * uz;
-        A += wl * ((C)1 - (C)4.5 * cu * cu + (C)1.5 * usq);
-        const C s = wl * ((C)3 + (C)9 * cu);
-        Bx += s * (C)cx(q);
-        By += s * (C)cy(q);
-        Bz += s * (C)cz(q);
-        Sw += wl;
+    for (int q = 0; q < 19; q++) Mout[(size_t)q * N + c] = mu[q];
+}
+
+// Two adjoint steps in one pass (the two-step pass of bgk_collide.h with the
+// pull mirrored: lambda is read at x + c_q): M^{t+1} -> M^t (u^t from tape0)
+// kept in LDS -> M^{t-1} (u^{t-1} from tape1).  Bit-identical to two
+// k_adjoint_step launches.
+template <typename C>
+struct AdjointOp {
+    static constexpr int SGN = -1;
+    struct Extra {
+        C u[3];
+    };
+    C omega;
+    const C *tape0, *tape1;
+    size_t ld;
+    __device__ __forceinline__ Extra extra_first(size_t i) const {
+        return Extra{{tape0[i], tape0[ld + i], tape0[2 * ld + i]}};
     }
-    const C Gx = Bx - (C)3 * Sw * ux, Gy = By - (C)3 * Sw * uy, Gz = Bz - (C)3 * Sw * uz;
-#pragma unroll
-    for (int q = 0; q < 19; q++)
-        Mout[(size_t)q * N + c] = ((C)1 - omega) * lam[q] +
-                                  omega * (A + Gx * (C)cx(q) + Gy * (C)cy(q) + Gz * (C)cz(q));
+    __device__ __forceinline__ Extra extra_second(size_t i) const {
+        return Extra{{tape1[i], tape1[ld + i], tape1[2 * ld + i]}};
+    }
+    __device__ __forceinline__ void first(const C (&f)[19], C (&b)[19], const Extra &e, size_t,
+                                          bool) const {
+        adjoint_site<C>(f, e.u[0], e.u[1], e.u[2], omega, b);
+    }
+    __device__ __forceinline__ void second(const C (&f)[19], C (&b)[19], const Extra &e,
+                                           size_t) const {
+        adjoint_site<C>(f, e.u[0], e.u[1], e.u[2], omega, b);
+    }
+};
+
+template <typename C, bool O32, int TX, int TY, int MINB>
+__global__ __launch_bounds__(collide2_threads(TX, TY), MINB) void k_adjoint2(Pops<C> src,
+                                                                             Pops<C> dst, Geo2 g,
+                                                                             AdjointOp<C> op) {
+    __shared__ C lds[20][TY + 2][TX + 2];
+    two_step_pass<C, C, O32, TX, TY>(src, dst, g, op, lds);
 }
 
 // dJ/du0 = sum_i lambda^0_i w_i (3 c_i + 9 (c_i.u0) c_i - 3 u0),
@@ -197,6 +243,83 @@ hipError_t launch_tape_step(bgk_ctx *c, C *tape) {
     return hipGetLastError();
 }
 
+// two taped forward steps (slices tape0, tape1) in one pass
+template <typename S, typename C, bool O32>
+hipError_t launch_tape2_o(bgk_ctx *c, C *tape0, C *tape1) {
+    const Lattice &L = c->lat;
+    using T = Tile2<S, C, O32>;
+    int nblocks;
+    const Geo2 g = make_geo2(L, T::X, T::Y, two_step_zc(L, T::X, T::Y, T::ZC), &nblocks);
+    const CollideOp<C, true> op{(C)c->omega, (C)1, tape0, tape1, (size_t)L.nsite};
+    hipLaunchKernelGGL((k_collide2<S, C, O32, T::X, T::Y, T::MINB, true>), dim3(nblocks),
+                       dim3(collide2_threads(T::X, T::Y)), 0, c->s0, pops_of<S>(L, c->cur),
+                       pops_of<S>(L, 1 - c->cur), g, op);
+    c->cur ^= 1;
+    return hipGetLastError();
+}
+
+// taped forward steps: slice k of `tape` records step k of the n steps
+template <typename S, typename C>
+int run_taped(bgk_ctx *c, C *tape, int n) {
+    const Lattice &L = c->lat;
+    const size_t slice = (size_t)3 * L.nsite;
+    int k = 0;
+    if (two_step_ok(L)) {
+        const bool o32 = offsets_fit_32(L, sizeof(S));
+        for (; k + 1 < n; k += 2)
+            BGK_HIP(o32 ? (launch_tape2_o<S, C, true>(c, tape + k * slice, tape + (k + 1) * slice))
+                        : (launch_tape2_o<S, C, false>(c, tape + k * slice,
+                                                       tape + (k + 1) * slice)));
+    }
+    for (; k < n; k++) BGK_HIP((launch_tape_step<S, C>(c, tape + k * slice)));
+    return BGK_OK;
+}
+
+template <typename C, bool O32>
+hipError_t launch_adjoint2_o(bgk_ctx *c, const C *lin, C *lout, const C *tape0,
+                             const C *tape1) {
+    const Lattice &L = c->lat;
+    using T = Tile2<C, C, O32>;
+    int nblocks;
+    const Geo2 g = make_geo2(L, T::X, T::Y, two_step_zc(L, T::X, T::Y, T::ZC), &nblocks);
+    Pops<C> a, b;
+    for (int q = 0; q < 19; q++) {
+        a.p[q] = (C *)lin + (size_t)q * L.nsite;
+        b.p[q] = lout + (size_t)q * L.nsite;
+    }
+    const AdjointOp<C> op{(C)c->omega, tape0, tape1, (size_t)L.nsite};
+    hipLaunchKernelGGL((k_adjoint2<C, O32, T::X, T::Y, T::MINB>), dim3(nblocks),
+                       dim3(collide2_threads(T::X, T::Y)), 0, c->s0, a, b, g, op);
+    return hipGetLastError();
+}
+
+template <typename C>
+hipError_t launch_adjoint_step(bgk_ctx *c, const C *lin, C *lout, const C *tape);
+
+// adjoint steps n-1 .. 0 of a segment whose tape slice k is step k; lin
+// holds M after step n and ends holding M after step 0
+template <typename C>
+int adjoint_sweep(bgk_ctx *c, C *&lin, C *&lout, const C *tape, int n) {
+    const Lattice &L = c->lat;
+    const size_t slice = (size_t)3 * L.nsite;
+    int k = n - 1;
+    if (two_step_ok(L)) {
+        const bool o32 = offsets_fit_32(L, sizeof(C));
+        for (; k >= 1; k -= 2) {
+            BGK_HIP(o32 ? (launch_adjoint2_o<C, true>(c, lin, lout, tape + k * slice,
+                                                      tape + (k - 1) * slice))
+                        : (launch_adjoint2_o<C, false>(c, lin, lout, tape + k * slice,
+                                                       tape + (k - 1) * slice)));
+            std::swap(lin, lout);
+        }
+    }
+    for (; k >= 0; k--) {
+        BGK_HIP(launch_adjoint_step<C>(c, lin, lout, tape + k * slice));
+        std::swap(lin, lout);
+    }
+    return BGK_OK;
+}
+
 template <typename C>
 hipError_t launch_adjoint_step(bgk_ctx *c, const C *lin, C *lout, const C *tape) {
     const Lattice &L = c->lat;
@@ -247,14 +370,15 @@ int adjoint_t(bgk_ctx *c, const double *u0, int T, int K, int levels, double *J,
     BGK_HIP(launch_init_equilibrium(L, c->cur, u0, N, 0, N, c->s0));
     for (int t = 0; t < T;) {
         if (t % K2 == 0 && (r = a.save(coarse[t / K2]))) return r;
-        if (t >= last_s0) {
-            BGK_HIP((launch_tape_step<S, C>(c, tape + (size_t)(t - last_s0) * 3 * N)));
-            t++;
-        } else {  // untaped steps up to the next checkpoint (two-step passes where they pay)
-            const int n = std::min(K2 - t % K2, last_s0 - t);
-            if ((r = step_periodic(c, n, false))) return r;
-            t += n;
-        }
+        // up to the next checkpoint; the last segment taped (two-step passes
+        // where they pay)
+        const int n = std::min(K2 - t % K2, (t >= last_s0 ? T : last_s0) - t);
+        if (t >= last_s0)
+            r = run_taped<S, C>(c, tape + (size_t)(t - last_s0) * 3 * N, n);
+        else
+            r = step_periodic(c, n, false);
+        if (r) return r;
+        t += n;
     }
     if ((r = reduce_ke(c, c->cur, J))) return r;  // synchronises
     hipLaunchKernelGGL((k_lambda_final<S, C>), dim3((unsigned)((N + 255) / 256)), dim3(256), 0,
@@ -293,15 +417,10 @@ int adjoint_t(bgk_ctx *c, const double *u0, int T, int K, int levels, double *J,
                 } else if ((r = a.restore(coarse[cs]))) {
                     return r;
                 }
-                for (int t = s0; t < s1; t++) {
-                    BGK_HIP((launch_tape_step<S, C>(c, tape + (size_t)(t - s0) * 3 * N)));
-                    a.recomputed++;
-                }
+                if ((r = run_taped<S, C>(c, tape, s1 - s0))) return r;
+                a.recomputed += s1 - s0;
             }
-            for (int t = s1 - 1; t >= s0; t--) {
-                BGK_HIP(launch_adjoint_step<C>(c, lin, lout, tape + (size_t)(t - s0) * 3 * N));
-                std::swap(lin, lout);
-            }
+            if ((r = adjoint_sweep<C>(c, lin, lout, tape, s1 - s0))) return r;
         }
     }
     hipLaunchKernelGGL(k_grad_u0<C>, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, c->s0, lin,

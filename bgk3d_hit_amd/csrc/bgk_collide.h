@@ -183,7 +183,6 @@ template <typename S, typename C, bool O32>
 struct Tile2 {
     static constexpr bool wide = sizeof(C) == 4 && O32;
     static constexpr int X = wide ? 128 : 64, Y = 5, ZC = 64, MINB = 1;
-    static constexpr bool DBUF = false;  // double-buffered loads (register cost)
 };
 
 // Planes per workgroup of the two-step pass: ZC, shortened on small lattices
@@ -201,15 +200,26 @@ inline int two_step_zc(const Lattice &L, int tx, int ty, int zc_max) {
     return zc < TWO_STEP_MIN_ZC && zc < L.nz ? 0 : (int)zc;
 }
 
-// population group: 0 c_z = -1, 1 c_z = 0, 2 c_z = +1 (all with an x/y
-// component), 3 no x/y component
+// A pass reads population q of site x from x - SGN c_q: SGN = +1 is the
+// stream-collide (pull), SGN = -1 the adjoint step (which reads x + c_q).
+// Source plane offset of population q:
+template <int SGN>
+__host__ __device__ constexpr int src_dz(int q) {
+    return -SGN * cz(q);
+}
+// population group: 0 read from the newest plane (source dz = +1), 1 from the
+// plane before (dz = 0), 2 from two planes before (dz = -1), all with an x/y
+// component; 3 no x/y component
+template <int SGN>
 __host__ __device__ constexpr int pgroup(int q) {
-    return (cx(q) == 0 && cy(q) == 0) ? 3 : (cz(q) < 0 ? 0 : (cz(q) == 0 ? 1 : 2));
+    return (cx(q) == 0 && cy(q) == 0) ? 3
+                                      : (src_dz<SGN>(q) > 0 ? 0 : (src_dz<SGN>(q) == 0 ? 1 : 2));
 }
 // index of q within its group
+template <int SGN>
 __host__ __device__ constexpr int pslot(int q) {
     int k = 0;
-    for (int i = 0; i < q; i++) k += (pgroup(i) == pgroup(q)) ? 1 : 0;
+    for (int i = 0; i < q; i++) k += (pgroup<SGN>(i) == pgroup<SGN>(q)) ? 1 : 0;
     return k;
 }
 // first LDS plane of a group (group 2: plus 4 per slot)
@@ -248,15 +258,58 @@ __device__ __forceinline__ void lds_barrier() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
-template <typename S, typename C, bool O32, int TX, int TY, int MINB, bool DBUF>
-__global__ __launch_bounds__(collide2_threads(TX, TY), MINB) void k_collide2(Pops<S> src,
-                                                                             Pops<S> dst, Geo2 g,
-                                                                             C omega, C cte1) {
+// Per-site operations of the two-step pass.  `first` runs at every site of
+// the grown tile (element index i; own = the site belongs to this workgroup's
+// chunk and tile), `second` at the workgroup's own sites.  Extra holds inputs
+// loaded with a plane's populations (prefetched one plane ahead).
+
+// stream-collide; TAPE: record u = j/rho of both steps (slices tape0, tape1)
+template <typename C, bool TAPE>
+struct CollideOp {
+    static constexpr int SGN = 1;
+    struct Extra {};
+    C omega, cte1;
+    C *tape0, *tape1;
+    size_t ld;  // sites per tape slice component
+    __device__ __forceinline__ Extra extra_first(size_t) const { return {}; }
+    __device__ __forceinline__ Extra extra_second(size_t) const { return {}; }
+    __device__ __forceinline__ void first(const C (&f)[19], C (&b)[19], const Extra &, size_t i,
+                                          bool own) const {
+        C ux, uy, uz;
+        collide_site<C>(f, b, omega, cte1, ux, uy, uz);
+        if (TAPE && own) {
+            tape0[i] = ux;
+            tape0[ld + i] = uy;
+            tape0[2 * ld + i] = uz;
+        }
+    }
+    __device__ __forceinline__ void second(const C (&f)[19], C (&b)[19], const Extra &,
+                                           size_t i) const {
+        C ux, uy, uz;
+        collide_site<C>(f, b, omega, cte1, ux, uy, uz);
+        if (TAPE) {
+            tape1[i] = ux;
+            tape1[ld + i] = uy;
+            tape1[2 * ld + i] = uz;
+        }
+    }
+};
+
+// The body of a two-step pass from `src` into `dst` (see the comment above
+// Tile2).  Every thread runs the same number of iterations and barriers.
+template <typename S, typename C, bool O32, int TX, int TY, class Op>
+__device__ __forceinline__ void two_step_pass(const Pops<S> &src, const Pops<S> &dst,
+                                              const Geo2 &g, const Op &op,
+                                              S (*lds)[TY + 2][TX + 2]) {
     using O = off_t_<O32>;
+    using X_t = typename Op::Extra;
+    constexpr int SGN = Op::SGN;
     constexpr O es = sizeof(S);
     constexpr int EX = TX + 2, EY = TY + 2;
     constexpr int NMAIN = TX * EY;  // threads of the x range [0, TX) of every row
-    __shared__ S lds[20][EY][EX];
+    // the z-only populations: QN read from the newest plane, QO from two
+    // planes before, QR the rest population
+    constexpr int QN = SGN > 0 ? 14 : 5, QO = SGN > 0 ? 5 : 14, QR = 18;
 
     const int t = xcd_remap((int)blockIdx.x, (int)gridDim.x);
     const int tx = t % g.tiles_x;
@@ -296,21 +349,23 @@ __global__ __launch_bounds__(collide2_threads(TX, TY), MINB) void k_collide2(Pop
     const O plane = (O)g.nxy * es;
 
     auto wrapz = [&](int z) { return z < 0 ? z + g.nz : (z >= g.nz ? z - g.nz : z); };
-    auto load_plane = [&](int z, S(&raw)[19]) {  // z in [-1, nz]
+    auto site_of = [&](int p) { return (size_t)((X[1] + Y[1] + (O)p * plane) / es); };
+    auto load_plane = [&](int z, S(&raw)[19], X_t &xr) {  // z in [-1, nz]
         const int p = wrapz(z);
         const int zm = (p == 0) ? g.nz - 1 : p - 1;
         const int zp = (p == g.nz - 1) ? 0 : p + 1;
         const O Z[3] = {(O)zm * plane, (O)p * plane, (O)zp * plane};
 #pragma unroll
         for (int q = 0; q < 19; q++)
-            raw[q] = ld_at(src.p[q], X[1 - cx(q)] + Y[1 - cy(q)] + Z[1 - cz(q)]);
+            raw[q] = ld_at(src.p[q], X[1 - SGN * cx(q)] + Y[1 - SGN * cy(q)] + Z[1 - SGN * cz(q)]);
+        xr = op.extra_first(site_of(p));
     };
 
-    S r5_1 = S{}, r5_2 = S{}, r18_1 = S{};  // S1 of this column at planes z-1, z-2
-    const int nit = zb - za + 2;            // iterations z = za-1 .. zb
+    S rO_1 = S{}, rO_2 = S{}, rR_1 = S{};  // S1 of this column at planes z-1, z-2
+    const int nit = zb - za + 2;           // iterations z = za-1 .. zb
     // iteration `it`: S1 of plane z from cur (loaded one iteration earlier),
     // loads of plane z+1 into nxt, S2 of plane z-1
-    auto iteration = [&](int it, S(&cur)[19], S(&nxt)[19]) {
+    auto iteration = [&](int it, S(&cur)[19], X_t &xcur, S(&nxt)[19], X_t &xnxt) {
         const int z = za - 1 + it;
         const int pb = pbase(2) + 4 * (it & 1);  // slot of S1(z-2), then of S1(z)
         S s1[19];
@@ -325,41 +380,45 @@ __global__ __launch_bounds__(collide2_threads(TX, TY), MINB) void k_collide2(Pop
             // (a copy at the loop end would wait for the loads)
             asm volatile("" : "+v"(f[q]));
         }
-        if (it + 1 < nit) load_plane(z + 1, nxt);  // in flight during this iteration
+        const X_t x1 = xcur;
+        if (it + 1 < nit) load_plane(z + 1, nxt, xnxt);  // in flight during this iteration
+        const bool second = owner && it >= 2;
+        const int p2 = wrapz(z - 1);
+        X_t x2{};
+        if (second) x2 = op.extra_second(site_of(p2));
         if (active) {
             C b[19];
-            C ux, uy, uz;
-            collide_site<C>(f, b, omega, cte1, ux, uy, uz);
+            op.first(f, b, x1, site_of(wrapz(z)), owner && z >= za && z < zb);
 #pragma unroll
             for (int q = 0; q < 19; q++) s1[q] = Storage<S>::template store<C>(b[q]);
 #pragma unroll
             for (int q = 0; q < 19; q++)
-                if (pgroup(q) == 0) lds[pbase(0) + pslot(q)][ey][ex] = s1[q];
+                if (pgroup<SGN>(q) == 0) lds[pbase(0) + pslot<SGN>(q)][ey][ex] = s1[q];
         }
         lds_barrier();
-        if (owner && it >= 2) {  // second step at plane z-1
-            C f[19];
+        if (second) {  // second step at plane z-1
+            C f2[19];
 #pragma unroll
             for (int q = 0; q < 19; q++) {
                 S v;
-                if (pgroup(q) == 0)
-                    v = lds[pbase(0) + pslot(q)][ey - cy(q)][ex - cx(q)];  // S1(z)
-                else if (pgroup(q) == 1)
-                    v = lds[pbase(1) + pslot(q)][ey - cy(q)][ex - cx(q)];  // S1(z-1)
-                else if (pgroup(q) == 2)
-                    v = lds[pb + pslot(q)][ey - cy(q)][ex - cx(q)];  // S1(z-2)
-                else if (cz(q) > 0)
-                    v = r5_2;
-                else if (cz(q) < 0)
+                const int sy = ey - SGN * cy(q), sx = ex - SGN * cx(q);
+                if (pgroup<SGN>(q) == 0)
+                    v = lds[pbase(0) + pslot<SGN>(q)][sy][sx];  // S1(z)
+                else if (pgroup<SGN>(q) == 1)
+                    v = lds[pbase(1) + pslot<SGN>(q)][sy][sx];  // S1(z-1)
+                else if (pgroup<SGN>(q) == 2)
+                    v = lds[pb + pslot<SGN>(q)][sy][sx];  // S1(z-2)
+                else if (q == QO)
+                    v = rO_2;
+                else if (q == QN)
                     v = s1[q];
                 else
-                    v = r18_1;
-                f[q] = Storage<S>::template load<C>(v);
+                    v = rR_1;
+                f2[q] = Storage<S>::template load<C>(v);
             }
             C b[19];
-            C ux, uy, uz;
-            collide_site<C>(f, b, omega, cte1, ux, uy, uz);
-            const O c = X[1] + Y[1] + (O)wrapz(z - 1) * plane;
+            op.second(f2, b, x2, site_of(p2));
+            const O c = X[1] + Y[1] + (O)p2 * plane;
 #pragma unroll
             for (int q = 0; q < 19; q++) st_at(dst.p[q], c, Storage<S>::template store<C>(b[q]));
         }
@@ -367,26 +426,26 @@ __global__ __launch_bounds__(collide2_threads(TX, TY), MINB) void k_collide2(Pop
         if (active) {
 #pragma unroll
             for (int q = 0; q < 19; q++) {
-                if (pgroup(q) == 1) lds[pbase(1) + pslot(q)][ey][ex] = s1[q];
-                if (pgroup(q) == 2) lds[pb + pslot(q)][ey][ex] = s1[q];
+                if (pgroup<SGN>(q) == 1) lds[pbase(1) + pslot<SGN>(q)][ey][ex] = s1[q];
+                if (pgroup<SGN>(q) == 2) lds[pb + pslot<SGN>(q)][ey][ex] = s1[q];
             }
         }
-        r5_2 = r5_1;
-        r5_1 = s1[5];
-        r18_1 = s1[18];
+        rO_2 = rO_1;
+        rO_1 = s1[QO];
+        rR_1 = s1[QR];
     };
 
     S ra[19];
-    load_plane(za - 1, ra);
-    if (DBUF) {  // two register sets for the loads in flight, alternating
-        S rb[19];
-        for (int it = 0; it < nit; it += 2) {
-            iteration(it, ra, rb);
-            if (it + 1 < nit) iteration(it + 1, rb, ra);
-        }
-    } else {  // one set: the values are copied out before the next loads
-        for (int it = 0; it < nit; it++) iteration(it, ra, ra);
-    }
+    X_t xa;
+    load_plane(za - 1, ra, xa);
+    for (int it = 0; it < nit; it++) iteration(it, ra, xa, ra, xa);
+}
+
+template <typename S, typename C, bool O32, int TX, int TY, int MINB, bool TAPE>
+__global__ __launch_bounds__(collide2_threads(TX, TY), MINB) void k_collide2(
+    Pops<S> src, Pops<S> dst, Geo2 g, CollideOp<C, TAPE> op) {
+    __shared__ S lds[20][TY + 2][TX + 2];
+    two_step_pass<S, C, O32, TX, TY>(src, dst, g, op, lds);
 }
 
 }  // namespace bgk

@@ -46,9 +46,10 @@ static hipError_t collide2_o(const Lattice &L, int src, double omega, hipStream_
     if (zc <= 0) return hipErrorInvalidValue;  // caller checks two_step_ok first
     int nblocks;
     Geo2 g = make_geo2(L, T::X, T::Y, zc, &nblocks);
-    hipLaunchKernelGGL((k_collide2<S, C, O32, T::X, T::Y, T::MINB, T::DBUF>), dim3(nblocks),
+    const CollideOp<C, false> op{(C)omega, (C)(L.shifted ? 1.0 : 0.0), nullptr, nullptr, 0};
+    hipLaunchKernelGGL((k_collide2<S, C, O32, T::X, T::Y, T::MINB, false>), dim3(nblocks),
                        dim3(collide2_threads(T::X, T::Y)), 0, s, pops_of<S>(L, src),
-                       pops_of<S>(L, 1 - src), g, (C)omega, (C)(L.shifted ? 1.0 : 0.0));
+                       pops_of<S>(L, 1 - src), g, op);
     return hipGetLastError();
 }
 // two steps from copy `src` into copy 1-src (periodic layout, separate rest
