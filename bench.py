@@ -306,7 +306,15 @@ def main():
                      "kernel": main_run["kernel"],
                      "steps_per_launch": main_run["steps_per_launch"],
                      "avg_launch_ms": round(main_run["collide_avg_ms"], 4),
-                     "algo_bytes_per_launch": main_run["algo_bytes_per_launch"]},
+                     "algo_bytes_per_launch": main_run["algo_bytes_per_launch"],
+                     # 38*s bytes per site-update at the measured kernel rate: what a
+                     # one-step-per-pass kernel would have to stream to match it
+                     "one_step_equivalent_gbs": round(
+                         38 * BYTES["f32"] * a.size ** 3 / (main_run["collide_ms_per_step"]
+                                                          * 1e-3) / 1e9, 1),
+                     "one_step_equivalent_frac": round(
+                         38 * BYTES["f32"] * a.size ** 3 / (main_run["collide_ms_per_step"]
+                                                          * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
         "cpu_baseline": cpu,
         "variants": variants,
         "configs": configs,
