@@ -227,6 +227,61 @@ static int step_slab(bgk_ctx *c, int nsteps) {
     return BGK_OK;
 }
 
+// Two steps per pass on the slab.  Stream s0: the two-step pass over the
+// bulk planes 2..nz-3 (it reads planes 0..nz-1 of the source copy only).
+// Stream s1: the halo exchange of the source copy (the single step's
+// message), the first step of planes 0..2 and nz-3..nz-1 into the plane
+// buffer slab_s1 (interior plane z at buffer plane z+1, resp. z+7-nz), the
+// exchange of the buffer's ghost planes 0 and 7 (again the 5 crossing
+// populations per face), and the second step of planes 0, 1, nz-2, nz-1 into
+// the destination copy.  Cross-stream dependencies per pair k:
+//   s1 first step(k) reads planes 2, 3, nz-4, nz-3 written by s0 bulk(k-1);
+//   s0 bulk(k) reads planes 0, 1, nz-2, nz-1 written by s1 boundary(k-1) and
+//   overwrites the copy that s1 boundary(k-1) read.
+static int step_slab2(bgk_ctx *c, int npairs) {
+    const Lattice &L = c->lat;
+    const int nz = L.nz;
+    const Box lo3{0, L.nx, 0, L.ny, 0, 3}, hi3{0, L.nx, 0, L.ny, nz - 3, nz};
+    const Box lo2{0, L.nx, 0, L.ny, 0, 2}, hi2{0, L.nx, 0, L.ny, nz - 2, nz};
+    BGK_HIP(hipEventRecord(c->ev0, c->s0));
+    BGK_HIP(hipStreamWaitEvent(c->s1, c->ev0, 0));
+    BGK_HIP(hipEventRecord(c->ev1, c->s1));
+    for (int k = 0; k < npairs; k++) {
+        const int a = c->cur, b = 1 - c->cur;
+        int r = halo_exchange(c, c->s1);  // does not depend on s0's previous pass
+        if (r) return r;
+        BGK_HIP(hipStreamWaitEvent(c->s1, c->ev0, 0));  // bulk(k-1) done
+        BGK_HIP(hipStreamWaitEvent(c->s0, c->ev1, 0));  // boundary(k-1) done
+        hipEvent_t ev;
+        BGK_HIP(timing_begin(c, TCLS_COLLIDE, c->s0, &ev));
+        BGK_HIP(launch_collide2(L, a, c->omega, c->s0));
+        BGK_HIP(timing_end(c, TCLS_COLLIDE, c->s0, ev));
+        BGK_HIP(timing_begin(c, TCLS_COLLIDE, c->s1, &ev));
+        BGK_HIP(launch_collide_planes(L, L.pop[a], L.zoff, c->slab_s1_pop, 1, lo3, c->omega,
+                                      c->s1));
+        BGK_HIP(launch_collide_planes(L, L.pop[a], L.zoff, c->slab_s1_pop, 7 - nz, hi3,
+                                      c->omega, c->s1));
+        BGK_HIP(timing_end(c, TCLS_COLLIDE, c->s1, ev));
+        BGK_HIP(timing_begin(c, TCLS_HALO, c->s1, &ev));
+        BGK_HIP(launch_pack_planes(L, c->slab_s1_pop, 6, 1, c->halo_send, c->s1));
+        BGK_HIP(launch_unpack_planes(L, c->slab_s1_pop, 0, 7, c->halo_send, c->s1));
+        BGK_HIP(timing_end(c, TCLS_HALO, c->s1, ev));
+        BGK_HIP(timing_begin(c, TCLS_COLLIDE, c->s1, &ev));
+        BGK_HIP(launch_collide_planes(L, c->slab_s1_pop, 1, L.pop[b], L.zoff, lo2, c->omega,
+                                      c->s1));
+        BGK_HIP(launch_collide_planes(L, c->slab_s1_pop, 7 - nz, L.pop[b], L.zoff, hi2,
+                                      c->omega, c->s1));
+        BGK_HIP(timing_end(c, TCLS_COLLIDE, c->s1, ev));
+        BGK_HIP(hipEventRecord(c->ev0, c->s0));
+        BGK_HIP(hipEventRecord(c->ev1, c->s1));
+        c->cur ^= 1;
+        c->steps += 2;
+    }
+    // join: later work on s0 sees the boundary planes
+    BGK_HIP(hipStreamWaitEvent(c->s0, c->ev1, 0));
+    return BGK_OK;
+}
+
 int reduce_ke(bgk_ctx *c, int copy, double *J) {
     BGK_HIP(launch_diagno(c->lat, copy, c->red_partials, c->red_out, c->s0));
     BGK_HIP(hipMemcpyAsync(c->red_host, c->red_out, 8 * sizeof(double), hipMemcpyDeviceToHost,
@@ -318,8 +373,7 @@ int bgk_create(int lx, int ly, int lz, int storage, int layout, int flags, doubl
     };
     // Two-step fusion reads the source copy's rest population at grown-tile
     // sites that another workgroup writes, so it needs a rest array per copy.
-    L.two_step = (layout == BGK_LAYOUT_PERIODIC &&
-                  !(flags & (BGK_FLAG_REFCOMPAT_HALOS | BGK_FLAG_SINGLE_STEP)))
+    L.two_step = !(flags & (BGK_FLAG_REFCOMPAT_HALOS | BGK_FLAG_SINGLE_STEP))
                      ? ((flags & BGK_FLAG_TWO_STEP_ANY_SIZE) ? 2 : 1)
                      : 0;
     const int nplanes = L.two_step ? 38 : 37;
@@ -350,6 +404,15 @@ int bgk_create(int lx, int ly, int lz, int storage, int layout, int flags, doubl
         if ((e = hipMalloc(&c->halo_send, c->halo_bytes)) != hipSuccess)
             return bail(hip_fail(e, "hipMalloc(halo)"));
         c->bytes += c->halo_bytes;
+        if (L.two_step) {  // first-step plane buffer of the two-step boundary path
+            const size_t pb8 = ((size_t)8 * lx * ly * es + 255) & ~(size_t)255;
+            if ((e = hipMalloc(&c->slab_s1, pb8 * 19)) != hipSuccess)
+                return bail(hip_fail(e, "hipMalloc(slab plane buffer)"));
+            c->bytes += pb8 * 19;
+            for (int q = 0; q < 19; q++) c->slab_s1_pop[q] = (char *)c->slab_s1 + pb8 * q;
+            if ((e = hipMemsetAsync(c->slab_s1, 0, pb8 * 19, c->s0)) != hipSuccess)
+                return bail(hip_fail(e, "hipMemsetAsync(slab plane buffer)"));
+        }
     }
     if (flags & BGK_FLAG_REFCOMPAT_HALOS) {
         c->gcache_bytes =
@@ -395,6 +458,7 @@ int bgk_destroy(bgk_ctx *c) {
     if (c->s1) (void)hipStreamDestroy(c->s1);
     (void)hipFree(c->pop_alloc);
     (void)hipFree(c->halo_send);
+    (void)hipFree(c->slab_s1);
     (void)hipFree(c->gcache[0]);
     (void)hipFree(c->gcache[1]);
     (void)hipFree(c->red_partials);
@@ -552,7 +616,16 @@ int bgk_step(bgk_ctx *c, int nsteps) {
     if (nsteps < 0) return fail(BGK_EINVAL, "nsteps must be >= 0");
     hipEvent_t ev;
     BGK_HIP(timing_begin(c, TCLS_STEP, c->s0, &ev));
-    r = (c->lat.layout == BGK_LAYOUT_SLAB_Z) ? step_slab(c, nsteps) : step_periodic(c, nsteps);
+    if (c->lat.layout == BGK_LAYOUT_SLAB_Z) {
+        r = BGK_OK;
+        if (two_step_ok(c->lat)) {
+            r = step_slab2(c, nsteps / 2);
+            nsteps %= 2;
+        }
+        if (!r) r = step_slab(c, nsteps);
+    } else {
+        r = step_periodic(c, nsteps);
+    }
     if (r) return r;
     BGK_HIP(timing_end(c, TCLS_STEP, c->s0, ev));
     return BGK_OK;

@@ -191,13 +191,18 @@ struct Tile2 {
 // would cost more than the pass saves: take single steps).
 constexpr int TWO_STEP_MIN_BLOCKS = 1024, TWO_STEP_MIN_ZC = 16;
 // L.two_step == 2 takes chunks of at least min(16, nz) planes at any size.
-inline int two_step_zc(const Lattice &L, int tx, int ty, int zc_max) {
+// nz: the number of second-step planes of the pass (the slab's bulk passes
+// cover nz - 4 of its planes).
+inline int two_step_zc(const Lattice &L, int tx, int ty, int zc_max, int nz) {
     const long long txy = (long long)((L.nx + tx - 1) / tx) * ((L.ny + ty - 1) / ty);
-    long long zc = txy * L.nz / TWO_STEP_MIN_BLOCKS;
+    long long zc = txy * nz / TWO_STEP_MIN_BLOCKS;
     if (L.two_step == 2 && zc < TWO_STEP_MIN_ZC) zc = TWO_STEP_MIN_ZC;
     if (zc > zc_max) zc = zc_max;
-    if (zc > L.nz) zc = L.nz;
-    return zc < TWO_STEP_MIN_ZC && zc < L.nz ? 0 : (int)zc;
+    if (zc > nz) zc = nz;
+    return zc < TWO_STEP_MIN_ZC && zc < nz ? 0 : (int)zc;
+}
+inline int two_step_zc(const Lattice &L, int tx, int ty, int zc_max) {
+    return two_step_zc(L, tx, ty, zc_max, L.layout == BGK_LAYOUT_PERIODIC ? L.nz : L.nz - 4);
 }
 
 // A pass reads population q of site x from x - SGN c_q: SGN = +1 is the
@@ -225,9 +230,14 @@ __host__ __device__ constexpr int pslot(int q) {
 // first LDS plane of a group (group 2: plus 4 per slot)
 __host__ __device__ constexpr int pbase(int grp) { return grp == 0 ? 0 : (grp == 1 ? 4 : 12); }
 
+// Geometry of a two-step pass: second-step planes [zlo, zhi) in chunks of
+// zc; periodic in z (wrap in the index math) or a slab whose bulk pass reads
+// only planes zlo-2 .. zhi+1, all inside the slab; interior plane z is stored
+// at z + zoff.
 struct Geo2 {
     int nx, ny, nz, nxy;
     int tiles_x, tiles_y, zc;
+    int zlo, zhi, zoff, periodic_z;
 };
 
 static Geo2 make_geo2(const Lattice &L, int tx, int ty, int zc, int *nblocks) {
@@ -239,7 +249,11 @@ static Geo2 make_geo2(const Lattice &L, int tx, int ty, int zc, int *nblocks) {
     g.tiles_x = (L.nx + tx - 1) / tx;
     g.tiles_y = (L.ny + ty - 1) / ty;
     g.zc = zc;
-    *nblocks = g.tiles_x * g.tiles_y * ((L.nz + g.zc - 1) / g.zc);
+    g.periodic_z = L.layout == BGK_LAYOUT_PERIODIC;
+    g.zoff = L.zoff;
+    g.zlo = g.periodic_z ? 0 : 2;
+    g.zhi = g.periodic_z ? L.nz : L.nz - 2;
+    *nblocks = g.tiles_x * g.tiles_y * ((g.zhi - g.zlo + g.zc - 1) / g.zc);
     return g;
 }
 
@@ -315,8 +329,8 @@ __device__ __forceinline__ void two_step_pass(const Pops<S> &src, const Pops<S> 
     const int tx = t % g.tiles_x;
     const int r = t / g.tiles_x;
     const int ty = r % g.tiles_y;
-    const int za = (r / g.tiles_y) * g.zc;
-    const int zb = min(g.nz, za + g.zc);
+    const int za = g.zlo + (r / g.tiles_y) * g.zc;
+    const int zb = min(g.zhi, za + g.zc);
 
     // site of the grown tile handled by this thread: (ex, ey) in [0,EX) x [0,EY);
     // the last wave takes the columns ex = 0 and ex = EX-1
@@ -348,13 +362,17 @@ __device__ __forceinline__ void two_step_pass(const Pops<S> &src, const Pops<S> 
     const O Y[3] = {(O)(ym * g.nx) * es, (O)(gy * g.nx) * es, (O)(yp * g.nx) * es};
     const O plane = (O)g.nxy * es;
 
-    auto wrapz = [&](int z) { return z < 0 ? z + g.nz : (z >= g.nz ? z - g.nz : z); };
-    auto site_of = [&](int p) { return (size_t)((X[1] + Y[1] + (O)p * plane) / es); };
-    auto load_plane = [&](int z, S(&raw)[19], X_t &xr) {  // z in [-1, nz]
+    // z in [zlo-1, zhi]; a slab's planes need no wrap
+    auto wrapz = [&](int z) {
+        return !g.periodic_z ? z : (z < 0 ? z + g.nz : (z >= g.nz ? z - g.nz : z));
+    };
+    auto site_of = [&](int p) { return (size_t)((X[1] + Y[1] + (O)(p + g.zoff) * plane) / es); };
+    auto load_plane = [&](int z, S(&raw)[19], X_t &xr) {
         const int p = wrapz(z);
-        const int zm = (p == 0) ? g.nz - 1 : p - 1;
-        const int zp = (p == g.nz - 1) ? 0 : p + 1;
-        const O Z[3] = {(O)zm * plane, (O)p * plane, (O)zp * plane};
+        const int zm = !g.periodic_z ? p - 1 : ((p == 0) ? g.nz - 1 : p - 1);
+        const int zp = !g.periodic_z ? p + 1 : ((p == g.nz - 1) ? 0 : p + 1);
+        const O Z[3] = {(O)(zm + g.zoff) * plane, (O)(p + g.zoff) * plane,
+                        (O)(zp + g.zoff) * plane};
 #pragma unroll
         for (int q = 0; q < 19; q++)
             raw[q] = ld_at(src.p[q], X[1 - SGN * cx(q)] + Y[1 - SGN * cy(q)] + Z[1 - SGN * cz(q)]);
@@ -418,7 +436,7 @@ __device__ __forceinline__ void two_step_pass(const Pops<S> &src, const Pops<S> 
             }
             C b[19];
             op.second(f2, b, x2, site_of(p2));
-            const O c = X[1] + Y[1] + (O)p2 * plane;
+            const O c = X[1] + Y[1] + (O)(p2 + g.zoff) * plane;
 #pragma unroll
             for (int q = 0; q < 19; q++) st_at(dst.p[q], c, Storage<S>::template store<C>(b[q]));
         }

@@ -72,6 +72,10 @@ struct bgk_ctx {
     // slab halo message: [face 0 = z+ (plane nz), 1 = z- (plane 1)][5][nx*ny]
     void *halo_send = nullptr;
     size_t halo_bytes = 0;
+    // slab two-step boundary path: first-step planes -1..2 and nz-3..nz
+    // stored as planes 0..7 of 19 arrays [8][nx*ny]
+    void *slab_s1 = nullptr;
+    void *slab_s1_pop[19] = {};
 
     // stale-halo images (reference ghost semantics), per copy:
     // gx[19][ny*nz], gy[19][nx*nz], gz[19][nx*ny]
@@ -127,6 +131,17 @@ hipError_t launch_diagno(const Lattice &L, int cur, double *partials, double *ou
 hipError_t launch_dissipation(const Lattice &L, int cur, const void *gcache, double svisc,
                               double *partials, double *out, hipStream_t s);
 // slab halo path
+//   pack: +z message from stored plane top, -z message from stored plane bot;
+//   unpack: +z message into stored plane lo, -z message into stored plane hi
+hipError_t launch_pack_planes(const Lattice &L, void *const *pops, int top, int bot, void *send,
+                              hipStream_t s);
+hipError_t launch_unpack_planes(const Lattice &L, void *const *pops, int lo, int hi,
+                                const void *recv, hipStream_t s);
+// one step over a slab box between arbitrary population arrays: interior
+// plane z read from stored plane z + src_off of src, written to z + dst_off
+hipError_t launch_collide_planes(const Lattice &L, void *const *src, int src_off,
+                                 void *const *dst, int dst_off, const Box &box, double omega,
+                                 hipStream_t s);
 hipError_t launch_pack_z(const Lattice &L, int cur, void *send, hipStream_t s);
 hipError_t launch_unpack_z(const Lattice &L, int cur, const void *recv, hipStream_t s);
 // stale-halo images of the periodic layout

@@ -349,40 +349,83 @@ static hipError_t checksum_t(const Lattice &L, int cur, void *partials, int nblo
 __constant__ int c_zplus[5] = {3, 5, 6, 12, 17};   // 04 06 07 13 18: leave through +z
 __constant__ int c_zminus[5] = {1, 8, 10, 14, 15}; // 02 09 11 15 16: leave through -z
 
-// message layout: [face][5][nx*ny], face 0 = the +z message (top interior
-// plane nz), face 1 = the -z message (bottom interior plane 1)
+// message layout: [face][5][nx*ny], face 0 = the +z message (stored plane
+// `top`: the top interior plane), face 1 = the -z message (stored plane
+// `bot`: the bottom interior plane)
 template <typename S>
-__global__ __launch_bounds__(256) void k_pack_z(Pops<S> P, int nxy, int nz, S *send) {
+__global__ __launch_bounds__(256) void k_pack_z(Pops<S> P, int nxy, int top, int bot, S *send) {
     const int p = blockIdx.x * 256 + threadIdx.x;
     if (p >= nxy) return;
     const int q5 = blockIdx.y;  // 0..4
-    send[(size_t)(0 * 5 + q5) * nxy + p] = P.p[c_zplus[q5]][(size_t)nz * nxy + p];
-    send[(size_t)(1 * 5 + q5) * nxy + p] = P.p[c_zminus[q5]][(size_t)1 * nxy + p];
+    send[(size_t)(0 * 5 + q5) * nxy + p] = P.p[c_zplus[q5]][(size_t)top * nxy + p];
+    send[(size_t)(1 * 5 + q5) * nxy + p] = P.p[c_zminus[q5]][(size_t)bot * nxy + p];
 }
 
-// the +z message fills the bottom ghost plane 0, the -z message the top
-// ghost plane nz+1 (SRC/bcond.comm.step3.F90:97-106, 133-142)
+// the +z message fills the bottom ghost plane `lo`, the -z message the top
+// ghost plane `hi` (SRC/bcond.comm.step3.F90:97-106, 133-142)
 template <typename S>
-__global__ __launch_bounds__(256) void k_unpack_z(Pops<S> P, int nxy, int nz, const S *recv) {
+__global__ __launch_bounds__(256) void k_unpack_z(Pops<S> P, int nxy, int lo, int hi,
+                                                  const S *recv) {
     const int p = blockIdx.x * 256 + threadIdx.x;
     if (p >= nxy) return;
     const int q5 = blockIdx.y;
-    P.p[c_zplus[q5]][p] = recv[(size_t)(0 * 5 + q5) * nxy + p];
-    P.p[c_zminus[q5]][(size_t)(nz + 1) * nxy + p] = recv[(size_t)(1 * 5 + q5) * nxy + p];
+    P.p[c_zplus[q5]][(size_t)lo * nxy + p] = recv[(size_t)(0 * 5 + q5) * nxy + p];
+    P.p[c_zminus[q5]][(size_t)hi * nxy + p] = recv[(size_t)(1 * 5 + q5) * nxy + p];
 }
 
 template <typename S>
-static hipError_t pack_z_t(const Lattice &L, int cur, void *send, hipStream_t s) {
+static Pops<S> pops_at(void *const *base) {
+    Pops<S> P;
+    for (int q = 0; q < 19; q++) P.p[q] = (S *)base[q];
+    return P;
+}
+template <typename S>
+static hipError_t pack_planes_t(const Lattice &L, void *const *pops, int top, int bot,
+                                void *send, hipStream_t s) {
     const int nxy = L.nx * L.ny;
     hipLaunchKernelGGL((k_pack_z<S>), dim3((nxy + 255) / 256, 5), dim3(256), 0, s,
-                       pops_of<S>(L, cur), nxy, L.nz, (S *)send);
+                       pops_at<S>(pops), nxy, top, bot, (S *)send);
     return hipGetLastError();
 }
 template <typename S>
-static hipError_t unpack_z_t(const Lattice &L, int cur, const void *recv, hipStream_t s) {
+static hipError_t unpack_planes_t(const Lattice &L, void *const *pops, int lo, int hi,
+                                  const void *recv, hipStream_t s) {
     const int nxy = L.nx * L.ny;
     hipLaunchKernelGGL((k_unpack_z<S>), dim3((nxy + 255) / 256, 5), dim3(256), 0, s,
-                       pops_of<S>(L, cur), nxy, L.nz, (const S *)recv);
+                       pops_at<S>(pops), nxy, lo, hi, (const S *)recv);
+    return hipGetLastError();
+}
+template <typename S>
+static hipError_t pack_z_t(const Lattice &L, int cur, void *send, hipStream_t s) {
+    return pack_planes_t<S>(L, L.pop[cur], L.nz, 1, send, s);
+}
+template <typename S>
+static hipError_t unpack_z_t(const Lattice &L, int cur, const void *recv, hipStream_t s) {
+    return unpack_planes_t<S>(L, L.pop[cur], 0, L.nz + 1, recv, s);
+}
+
+// one step over the slab box `box` from population arrays `src` into `dst`,
+// where interior plane z is stored at plane z + src_off (z + dst_off): the
+// slab's two-step boundary path, which takes single steps through a small
+// plane buffer
+template <typename S, typename C>
+static hipError_t collide_planes_t(const Lattice &L, void *const *src, int src_off,
+                                   void *const *dst, int dst_off, const Box &box, double omega,
+                                   hipStream_t s) {
+    if (box.sites() <= 0) return hipSuccess;
+    int ntiles;
+    Geo g = make_geo<S>(L, box, &ntiles);
+    g.zoff = 0;
+    g.periodic_z = 0;
+    const size_t nxy = (size_t)L.nx * L.ny;
+    Pops<S> a, b;
+    for (int q = 0; q < 19; q++) {
+        a.p[q] = (S *)src[q] + (ptrdiff_t)src_off * (ptrdiff_t)nxy;
+        b.p[q] = (S *)dst[q] + (ptrdiff_t)dst_off * (ptrdiff_t)nxy;
+    }
+    // 64-bit offsets: the shifted base pointers may lie below an array
+    hipLaunchKernelGGL((k_collide<S, C, false>), dim3(ntiles), dim3(Tile<S>::X, Tile<S>::Y), 0, s,
+                       a, b, g, (C)omega, (C)(L.shifted ? 1.0 : 0.0));
     return hipGetLastError();
 }
 
@@ -571,6 +614,8 @@ hipError_t launch_collide(const Lattice &L, int src, const Box &box, double omeg
 template <typename S, typename C>
 static bool two_step_ok_t(const Lattice &L) {
     if (!L.two_step) return false;
+    // slab: boundary planes 0..2 and nz-3..nz-1 must be distinct
+    if (L.layout == BGK_LAYOUT_SLAB_Z && L.nz < 6) return false;
     return offsets_fit_32(L, sizeof(S))
                ? two_step_zc(L, Tile2<S, C, true>::X, Tile2<S, C, true>::Y,
                              Tile2<S, C, true>::ZC) > 0
@@ -597,6 +642,19 @@ hipError_t launch_diagno(const Lattice &L, int cur, double *partials, double *ou
 hipError_t launch_dissipation(const Lattice &L, int cur, const void *gcache, double svisc,
                               double *partials, double *out, hipStream_t s) {
     BGK_DISPATCH(L.storage, dissipation_t, L, cur, gcache, svisc, partials, out, s);
+}
+hipError_t launch_pack_planes(const Lattice &L, void *const *pops, int top, int bot, void *send,
+                              hipStream_t s) {
+    BGK_DISPATCH_S(L.storage, pack_planes_t, L, pops, top, bot, send, s);
+}
+hipError_t launch_unpack_planes(const Lattice &L, void *const *pops, int lo, int hi,
+                                const void *recv, hipStream_t s) {
+    BGK_DISPATCH_S(L.storage, unpack_planes_t, L, pops, lo, hi, recv, s);
+}
+hipError_t launch_collide_planes(const Lattice &L, void *const *src, int src_off,
+                                 void *const *dst, int dst_off, const Box &box, double omega,
+                                 hipStream_t s) {
+    BGK_DISPATCH(L.storage, collide_planes_t, L, src, src_off, dst, dst_off, box, omega, s);
 }
 hipError_t launch_pack_z(const Lattice &L, int cur, void *send, hipStream_t s) {
     BGK_DISPATCH_S(L.storage, pack_z_t, L, cur, send, s);

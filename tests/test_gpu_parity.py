@@ -135,16 +135,18 @@ def test_masked_bulk_then_border_equals_full(gpu, kind):
 
 
 @pytest.mark.parametrize("kind", ["f32", "f64", "f16"])
-def test_slab_overlap_bitwise_equals_periodic(gpu, kind):
+@pytest.mark.parametrize("two_step", ["always", False])
+def test_slab_overlap_bitwise_equals_periodic(gpu, kind, two_step):
     """One rank's slab with +-z ghost planes, pack / self-exchange / unpack,
-    bulk and boundary collides on two streams: identical to the periodic
-    kernel on the same domain."""
+    bulk and boundary collides on two streams (one step per pass, or the
+    two-step pass on the bulk with the boundary planes stepped twice through
+    the plane buffer): identical to the periodic kernel on the same domain."""
     shape = (64, 48, 16)
-    say("slab", kind, shape)
+    say("slab", kind, shape, two_step)
     u = bgk.hit_velocity(*shape, 4.0, 0.03, 5678)
-    p = bgk.Lattice(*shape, omega=1.97, storage=kind)
-    s = bgk.Lattice(*shape, omega=1.97, storage=kind, layout="slab")
-    s2 = bgk.Lattice(*shape, omega=1.97, storage=kind, layout="slab")
+    p = bgk.Lattice(*shape, omega=1.97, storage=kind, two_step=False)
+    s = bgk.Lattice(*shape, omega=1.97, storage=kind, layout="slab", two_step=two_step)
+    s2 = bgk.Lattice(*shape, omega=1.97, storage=kind, layout="slab", two_step=two_step)
     for x in (p, s, s2):
         x.init_equilibrium(u)
     p.step(25)
@@ -158,6 +160,25 @@ def test_slab_overlap_bitwise_equals_periodic(gpu, kind):
     # consistent dissipation refreshes the ghosts first: same as periodic
     nu = bgk.svisc_from_omega(1.97)
     assert np.allclose(s.dissipation(nu), p.dissipation(nu), rtol=1e-13, atol=0)
+
+
+@pytest.mark.parametrize("kind", ["f32", "f32mix", "bf16"])
+@pytest.mark.parametrize("shape", [(70, 9, 6), (33, 20, 7), (64, 16, 40)])
+def test_slab_two_step_ragged(gpu, kind, shape):
+    """The slab's two-step path at the smallest slab it takes (6 planes),
+    odd plane counts and ragged x/y, odd and even step counts."""
+    say("slab two-step ragged", kind, shape)
+    u = bgk.hit_velocity(*shape, 2.0, 0.03, 13)
+    p = bgk.Lattice(*shape, omega=1.93, storage=kind, two_step=False)
+    s = bgk.Lattice(*shape, omega=1.93, storage=kind, layout="slab", two_step="always",
+                    timing=True)
+    p.init_equilibrium(u)
+    s.init_equilibrium(u)
+    for n in (2, 3, 1, 6):
+        p.step(n)
+        s.step(n)
+        assert np.array_equal(s.get_populations(), p.get_populations()), n
+    assert s.timing()["collide_launches"] < 12 * 3  # took two-step passes
 
 
 def test_slab_masked_path(gpu):
