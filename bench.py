@@ -197,6 +197,46 @@ def cpu_baseline(min_seconds=10.0, max_steps=400):
                       f"workload, {steps} steps in {wall:.1f} s"}
 
 
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+def cpu_samples(seconds=2.0):
+    """The oracle at the small configurations (64^3 FP64 as in the CPU-runnable
+    case, 128^3 FP32 and FP64), on all of its threads and on one thread, each
+    for about `seconds`; MLUPS per sample."""
+    import bgk3d_hit_amd as bgk
+    import oracle
+
+    nthreads = oracle.num_threads()
+    out = {"cpu_model": cpu_model(), "host_cpus": os.cpu_count(), "samples": []}
+    for n, kind in ((64, "f64"), (128, "f32"), (128, "f64")):
+        u = bgk.hit_velocity(n, n, n, 4.0, 0.02, 1234)
+        for threads in (nthreads, 1):
+            oracle.set_num_threads(threads)
+            o = oracle.OracleLattice(n, n, n, omega=1.90, kind=kind)
+            o.init_equilibrium(u)
+            o.step(1)
+            steps, t0 = 0, time.perf_counter()
+            while steps < 2 or (time.perf_counter() - t0) < seconds:
+                o.step(1)
+                steps += 1
+            wall = time.perf_counter() - t0
+            o.close()
+            out["samples"].append({"lattice": n, "kind": kind, "threads": threads,
+                                   "steps": steps,
+                                   "mlups": round(n ** 3 * steps / wall / 1e6, 2)})
+    oracle.set_num_threads(nthreads)
+    return out
+
+
 def calibrate_copy(gib, reps=10):
     """Plain copy of a known size: dst.copy_(src) moves gib GiB in and gib GiB out."""
     import torch
@@ -287,10 +327,13 @@ def main():
         log("[bench] adjoint 128^3")
         configs["adjoint_128cubed"] = guarded("adjoint", c5_config)
     cpu = None
+    cpu_more = None
     if not a.no_cpu:
         log("[bench] cpu baseline")
         cpu = guarded("cpu", cpu_baseline)
         log(f"[bench] cpu: {cpu}")
+        cpu_more = guarded("cpu samples", cpu_samples)
+        log(f"[bench] cpu samples: {cpu_more}")
     out = {
         "metric": METRIC,
         "value": round(main_run["mlups"], 1),
@@ -326,6 +369,7 @@ def main():
                          38 * BYTES["f32"] * a.size ** 3 / (main_run["collide_ms_per_step"]
                                                           * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
         "cpu_baseline": cpu,
+        "cpu_samples": cpu_more,
         "variants": variants,
         "configs": configs,
     }

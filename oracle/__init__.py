@@ -61,6 +61,7 @@ def load():
         "orc_f32_to_bf16": (ctypes.c_uint16, [ctypes.c_float]),
         "orc_bf16_to_f32": (ctypes.c_float, [ctypes.c_uint16]),
         "orc_num_threads": (ctypes.c_int, []),
+        "orc_set_num_threads": (None, [ctypes.c_int]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -159,3 +160,7 @@ def forward_ke(lx, ly, lz, omega, nsteps, u0):
 
 def num_threads():
     return load().orc_num_threads()
+
+
+def set_num_threads(n):
+    load().orc_set_num_threads(int(n))

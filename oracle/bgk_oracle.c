@@ -260,6 +260,16 @@ int orc_num_threads(void)
 #endif
 }
 
+/* threads for the following calls (timing the CPU baseline on 1 thread) */
+void orc_set_num_threads(int n)
+{
+#ifdef _OPENMP
+    if (n > 0) omp_set_num_threads(n);
+#else
+    (void)n;
+#endif
+}
+
 /* ---- discrete adjoint (not in the reference; FP64) -------------------
  * Forward map per step: g_k(x) = f_k(x - c_k) (pull), f'_i(x) = C_i(g(x))
  * with C_i = g_i - omega (g_i - e_i(g)).  This restatement applies J^T with

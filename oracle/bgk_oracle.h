@@ -95,6 +95,7 @@ uint16_t orc_f32_to_bf16(float f);
 float orc_bf16_to_f32(uint16_t h);
 
 int orc_num_threads(void);
+void orc_set_num_threads(int n);
 
 #ifdef __cplusplus
 }
