@@ -1,6 +1,9 @@
-// bgk_collide.h -- the fused pull stream + BGK collide kernel, shared by the
-// time-step path (bgk_kernels.hip) and the adjoint's taping forward pass
-// (bgk_adjoint.hip).  SRC/col_MC.F90:104-361.
+// bgk_collide.h -- the fused pull stream + BGK collide kernels
+// (SRC/col_MC.F90:104-361): k_collide, one time step per pass, and the
+// two-step pass (two_step_pass / k_collide2), which advances two time steps
+// per pass over HBM.  Shared by the time-step path (bgk_kernels.hip) and the
+// adjoint (bgk_adjoint.hip: taping forward, and the same two-step pass body
+// with the adjoint step as its per-site operation).
 #pragma once
 
 #include <type_traits>
@@ -159,22 +162,25 @@ __global__ __launch_bounds__(Tile<S>::X * Tile<S>::Y) void k_collide(Pops<S> src
 }
 
 // --------------------------------------------------------- two-step collide
-// Two time steps in one pass over HBM (temporal blocking; periodic layout).
-// A workgroup owns a TX x TY column of sites over a chunk of ZC planes and
-// marches it in z.  At plane z it computes the first step S1 on the tile
-// grown by one site on every x/y side, reading the source copy S0, keeps S1
-// in LDS, and computes the second step S2 of plane z-1 from there into the
-// destination copy.  S1 goes through the storage type exactly as a store and
-// reload would, so the result equals two single steps bit for bit.
+// Two time steps in one pass over HBM (temporal blocking).  A workgroup owns
+// a TX x TY column of sites over a chunk of ZC planes and marches it in z.
+// At plane z it computes the first step S1 on the tile grown by one site on
+// every x/y side, reading the source copy S0, keeps S1 in LDS, and computes
+// the second step S2 of plane z-1 from there into the destination copy.  S1
+// goes through the storage type exactly as a store and reload would, so the
+// result equals two single steps bit for bit.  x and y wrap periodically; z
+// wraps on the periodic lattice, and on a slab the pass covers the bulk
+// planes whose first step needs no ghost plane.
 //
 // Only the S1 values that a later neighbour pull reads are kept in LDS, each
-// for as long as it is needed (a population with c_z = d at plane p is read
-// by the second step at plane p + d):
+// for as long as it is needed (for the stream-collide a population with
+// c_z = d at plane p is read by the second step at plane p + d):
 //   c_z = -1 with an x/y component (4): plane z, read within the same iteration
 //   c_z =  0 with an x/y component (8): plane z-1, one slot
 //   c_z = +1 with an x/y component (4): planes z-1 and z-2, two slots
 // The populations (0,0,+-1) and the rest population move only in z; they stay
 // in registers of the thread that owns the column.
+//
 // Grown-tile shape (X x Y interior sites, ZC planes per workgroup, MINB
 // workgroups per CU), from a sweep at 512^3: 128 x 5 (960 threads) for FP32
 // compute with 32-bit offsets; 64 x 5 (512 threads, twice the registers per

@@ -1,8 +1,10 @@
 // bgk_kernels.hip -- gfx950 kernels of the BGK3D_HIT hot path.
 //
 //   k_collide          fused pull stream + BGK collide (SRC/col_MC.F90:104-361),
-//                      one thread per site, x-contiguous 64x4 tiles, periodic
-//                      wrap in the index math (or slab ghost planes in z)
+//                      one thread per site, x-contiguous tiles, periodic wrap
+//                      in the index math (or slab ghost planes in z)
+//   k_collide2         two time steps per pass (launchers here, kernel in
+//                      bgk_collide.h); on the slab for the bulk planes
 //   k_diagno           at-site moment sums (SRC/diagno.F90:73-115)
 //   k_dissipation      pulled Pi^neq / kinetic energy sums
 //                      (SRC/dissipation.F90:70-253)
