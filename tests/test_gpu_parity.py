@@ -262,6 +262,40 @@ def test_two_step_pass_equals_single_steps(gpu, kind, shape):
         assert np.array_equal(a.get_populations(), b.get_populations()), n
 
 
+def test_two_step_pass_512cubed_fp32_checksum(gpu):
+    """The headline lattice: 512^3 FP32, 10 steps with the two-step pass ==
+    10 single steps, bit for bit."""
+    say("two-step 512^3 f32")
+    a = bgk.Lattice(512, 512, 512, omega=1.95, storage="f32", timing=True)
+    a.init_hit(4.0, 0.02, 1234)
+    a.step(10)
+    ca = a.checksum()
+    assert a.timing()["collide_launches"] == 5
+    a.close()
+    b = bgk.Lattice(512, 512, 512, omega=1.95, storage="f32", two_step=False)
+    b.init_hit(4.0, 0.02, 1234)
+    b.step(10)
+    assert b.checksum() == ca
+
+
+def test_two_step_pass_64bit_offsets(gpu):
+    """A lattice whose population arrays exceed 4 GiB (1040^3 FP32, 4.5 GB per
+    array) takes the 64-bit-offset kernels; two-step passes == single steps."""
+    n = 1040
+    say("two-step 1040^3 f32, 64-bit offsets")
+    a = bgk.Lattice(n, n, n, omega=1.95, storage="f32", timing=True)
+    a.init_hit(4.0, 0.02, 77)
+    a.step(4)
+    ca = a.checksum()
+    assert a.timing()["collide_launches"] == 2
+    a.close()
+    b = bgk.Lattice(n, n, n, omega=1.95, storage="f32", two_step=False)
+    b.init_hit(4.0, 0.02, 77)
+    b.step(4)
+    assert b.checksum() == ca
+    b.close()
+
+
 @pytest.mark.parametrize("kind", ["f32", "f16", "f64"])
 def test_two_step_pass_256cubed_checksum(gpu, kind):
     """At 256^3 step() takes the two-step pass by itself (10 launches for 20
