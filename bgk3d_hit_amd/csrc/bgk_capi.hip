@@ -166,20 +166,34 @@ static int record_faces(bgk_ctx *c, hipStream_t s) {
     return BGK_OK;
 }
 
+// With reference halo images, each copy's image slot holds the faces the
+// reference's ghosts of that array would hold: after time T, slot[cur] the
+// faces of S_{T-2} and slot[1-cur] those of S_{T-1}.  A single step records
+// S_T into slot[cur] before it; a two-step pass from S_T (copy a) to S_{T+2}
+// (copy 1-a) records S_T into slot[1-a] and writes the faces of its
+// intermediate S_{T+1} into slot[a] from inside the pass.
 int step_periodic(bgk_ctx *c, int nsteps, bool count) {
     std::vector<Box> all{full_box(c->lat)};
+    const bool images = (c->flags & BGK_FLAG_REFCOMPAT_HALOS) != 0;
     const int n0 = nsteps;
     if (two_step_ok(c->lat)) {
         for (; nsteps >= 2; nsteps -= 2) {
+            const int a = c->cur;
             hipEvent_t ev;
+            if (images) {
+                BGK_HIP(timing_begin(c, TCLS_HALO, c->s0, &ev));
+                BGK_HIP(launch_pack_faces(c->lat, a, c->gcache[1 - a], c->s0));
+                BGK_HIP(timing_end(c, TCLS_HALO, c->s0, ev));
+            }
             BGK_HIP(timing_begin(c, TCLS_COLLIDE, c->s0, &ev));
-            BGK_HIP(launch_collide2(c->lat, c->cur, c->omega, c->s0));
+            BGK_HIP(launch_collide2(c->lat, a, c->omega, images ? c->gcache[a] : nullptr,
+                                    c->s0));
             BGK_HIP(timing_end(c, TCLS_COLLIDE, c->s0, ev));
             c->cur ^= 1;
         }
     }
     for (int t = 0; t < nsteps; t++) {
-        if (c->flags & BGK_FLAG_REFCOMPAT_HALOS) {
+        if (images) {
             int r = record_faces(c, c->s0);
             if (r) return r;
         }
@@ -254,7 +268,7 @@ static int step_slab2(bgk_ctx *c, int npairs) {
         BGK_HIP(hipStreamWaitEvent(c->s0, c->ev1, 0));  // boundary(k-1) done
         hipEvent_t ev;
         BGK_HIP(timing_begin(c, TCLS_COLLIDE, c->s0, &ev));
-        BGK_HIP(launch_collide2(L, a, c->omega, c->s0));
+        BGK_HIP(launch_collide2(L, a, c->omega, nullptr, c->s0));
         BGK_HIP(timing_end(c, TCLS_COLLIDE, c->s0, ev));
         BGK_HIP(timing_begin(c, TCLS_COLLIDE, c->s1, &ev));
         BGK_HIP(launch_collide_planes(L, L.pop[a], L.zoff, c->slab_s1_pop, 1, lo3, c->omega,
@@ -373,7 +387,7 @@ int bgk_create(int lx, int ly, int lz, int storage, int layout, int flags, doubl
     };
     // Two-step fusion reads the source copy's rest population at grown-tile
     // sites that another workgroup writes, so it needs a rest array per copy.
-    L.two_step = !(flags & (BGK_FLAG_REFCOMPAT_HALOS | BGK_FLAG_SINGLE_STEP))
+    L.two_step = !(flags & BGK_FLAG_SINGLE_STEP)
                      ? ((flags & BGK_FLAG_TWO_STEP_ANY_SIZE) ? 2 : 1)
                      : 0;
     const int nplanes = L.two_step ? 38 : 37;

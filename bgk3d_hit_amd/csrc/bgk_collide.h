@@ -244,6 +244,7 @@ struct Geo2 {
     int nx, ny, nz, nxy;
     int tiles_x, tiles_y, zc;
     int zlo, zhi, zoff, periodic_z;
+    void *faces;  // FACES: face images of the first step (k_pack_faces layout)
 };
 
 static Geo2 make_geo2(const Lattice &L, int tx, int ty, int zc, int *nblocks) {
@@ -259,6 +260,7 @@ static Geo2 make_geo2(const Lattice &L, int tx, int ty, int zc, int *nblocks) {
     g.zoff = L.zoff;
     g.zlo = g.periodic_z ? 0 : 2;
     g.zhi = g.periodic_z ? L.nz : L.nz - 2;
+    g.faces = nullptr;
     *nblocks = g.tiles_x * g.tiles_y * ((g.zhi - g.zlo + g.zc - 1) / g.zc);
     return g;
 }
@@ -317,7 +319,10 @@ struct CollideOp {
 
 // The body of a two-step pass from `src` into `dst` (see the comment above
 // Tile2).  Every thread runs the same number of iterations and barriers.
-template <typename S, typename C, bool O32, int TX, int TY, class Op>
+// FACES (periodic lattice): also write the face planes of the first step's
+// state to g.faces, in the layout of k_pack_faces, for the dissipation that
+// reads the stale halo images of the reference (BGK_FLAG_REFCOMPAT_HALOS).
+template <typename S, typename C, bool O32, int TX, int TY, class Op, bool FACES = false>
 __device__ __forceinline__ void two_step_pass(const Pops<S> &src, const Pops<S> &dst,
                                               const Geo2 &g, const Op &op,
                                               S (*lds)[TY + 2][TX + 2]) {
@@ -418,6 +423,22 @@ __device__ __forceinline__ void two_step_pass(const Pops<S> &src, const Pops<S> 
 #pragma unroll
             for (int q = 0; q < 19; q++)
                 if (pgroup<SGN>(q) == 0) lds[pbase(0) + pslot<SGN>(q)][ey][ex] = s1[q];
+            if (FACES && owner && z >= za && z < zb) {
+                // image of population q on the face it leaves through
+                const int p = wrapz(z);
+                S *fx = (S *)g.faces;
+                S *fy = fx + (size_t)19 * g.ny * g.nz;
+                S *fz = fy + (size_t)19 * g.nx * g.nz;
+#pragma unroll
+                for (int q = 0; q < 18; q++) {
+                    if (cx(q) != 0 && gx == (cx(q) > 0 ? g.nx - 1 : 0))
+                        fx[(size_t)q * g.ny * g.nz + gy + (size_t)g.ny * p] = s1[q];
+                    if (cy(q) != 0 && gy == (cy(q) > 0 ? g.ny - 1 : 0))
+                        fy[(size_t)q * g.nx * g.nz + gx + (size_t)g.nx * p] = s1[q];
+                    if (cz(q) != 0 && p == (cz(q) > 0 ? g.nz - 1 : 0))
+                        fz[(size_t)q * g.nxy + gx + (size_t)g.nx * gy] = s1[q];
+                }
+            }
         }
         lds_barrier();
         if (second) {  // second step at plane z-1
@@ -465,11 +486,12 @@ __device__ __forceinline__ void two_step_pass(const Pops<S> &src, const Pops<S> 
     for (int it = 0; it < nit; it++) iteration(it, ra, xa, ra, xa);
 }
 
-template <typename S, typename C, bool O32, int TX, int TY, int MINB, bool TAPE>
+template <typename S, typename C, bool O32, int TX, int TY, int MINB, bool TAPE,
+          bool FACES = false>
 __global__ __launch_bounds__(collide2_threads(TX, TY), MINB) void k_collide2(
     Pops<S> src, Pops<S> dst, Geo2 g, CollideOp<C, TAPE> op) {
     __shared__ S lds[20][TY + 2][TX + 2];
-    two_step_pass<S, C, O32, TX, TY>(src, dst, g, op, lds);
+    two_step_pass<S, C, O32, TX, TY, CollideOp<C, TAPE>, FACES>(src, dst, g, op, lds);
 }
 
 }  // namespace bgk

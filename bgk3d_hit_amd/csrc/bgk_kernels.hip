@@ -39,27 +39,33 @@ static hipError_t collide_t(const Lattice &L, int src, const Box &box, double om
     return hipGetLastError();
 }
 
-// two steps from copy `src` into copy 1-src (periodic layout, separate rest
-// arrays per copy)
 template <typename S, typename C, bool O32>
-static hipError_t collide2_o(const Lattice &L, int src, double omega, hipStream_t s) {
+static hipError_t collide2_o(const Lattice &L, int src, double omega, void *faces,
+                             hipStream_t s) {
     using T = Tile2<S, C, O32>;
     const int zc = two_step_zc(L, T::X, T::Y, T::ZC);
     if (zc <= 0) return hipErrorInvalidValue;  // caller checks two_step_ok first
     int nblocks;
     Geo2 g = make_geo2(L, T::X, T::Y, zc, &nblocks);
+    g.faces = faces;
     const CollideOp<C, false> op{(C)omega, (C)(L.shifted ? 1.0 : 0.0), nullptr, nullptr, 0};
-    hipLaunchKernelGGL((k_collide2<S, C, O32, T::X, T::Y, T::MINB, false>), dim3(nblocks),
-                       dim3(collide2_threads(T::X, T::Y)), 0, s, pops_of<S>(L, src),
-                       pops_of<S>(L, 1 - src), g, op);
+    if (faces)
+        hipLaunchKernelGGL((k_collide2<S, C, O32, T::X, T::Y, T::MINB, false, true>),
+                           dim3(nblocks), dim3(collide2_threads(T::X, T::Y)), 0, s,
+                           pops_of<S>(L, src), pops_of<S>(L, 1 - src), g, op);
+    else
+        hipLaunchKernelGGL((k_collide2<S, C, O32, T::X, T::Y, T::MINB, false>), dim3(nblocks),
+                           dim3(collide2_threads(T::X, T::Y)), 0, s, pops_of<S>(L, src),
+                           pops_of<S>(L, 1 - src), g, op);
     return hipGetLastError();
 }
 // two steps from copy `src` into copy 1-src (periodic layout, separate rest
 // arrays per copy)
 template <typename S, typename C>
-static hipError_t collide2_t(const Lattice &L, int src, double omega, hipStream_t s) {
-    return offsets_fit_32(L, sizeof(S)) ? collide2_o<S, C, true>(L, src, omega, s)
-                                        : collide2_o<S, C, false>(L, src, omega, s);
+static hipError_t collide2_t(const Lattice &L, int src, double omega, void *faces,
+                             hipStream_t s) {
+    return offsets_fit_32(L, sizeof(S)) ? collide2_o<S, C, true>(L, src, omega, faces, s)
+                                        : collide2_o<S, C, false>(L, src, omega, faces, s);
 }
 
 // --------------------------------------------------------------- reductions
@@ -634,8 +640,9 @@ bool two_step_ok(const Lattice &L) {
     default: return false;
     }
 }
-hipError_t launch_collide2(const Lattice &L, int src, double omega, hipStream_t s) {
-    BGK_DISPATCH(L.storage, collide2_t, L, src, omega, s);
+hipError_t launch_collide2(const Lattice &L, int src, double omega, void *faces,
+                           hipStream_t s) {
+    BGK_DISPATCH(L.storage, collide2_t, L, src, omega, faces, s);
 }
 hipError_t launch_diagno(const Lattice &L, int cur, double *partials, double *out,
                          hipStream_t s) {

@@ -135,9 +135,8 @@ int bgk_col_MC_masked(bgk_ctx *ctx, int opt);
 
 /* nsteps of the time loop body boundaries + col_MC (SRC/bgk3d_HIT.F90:127-146
  * without diagnostic).  Two steps per pass over the populations where the
- * lattice is large enough (unless BGK_FLAG_SINGLE_STEP or
- * BGK_FLAG_REFCOMPAT_HALOS), then single steps; bit-identical to nsteps
- * calls of bgk_col_MC.  Slab layout: the interior collide overlaps the halo
+ * lattice is large enough (unless BGK_FLAG_SINGLE_STEP), then single steps;
+ * bit-identical to nsteps calls of bgk_col_MC, stale halo images included.  Slab layout: the interior collide overlaps the halo
  * pack/exchange/unpack and boundary collide on a second HIP stream (the
  * structure of SRC/bcond.comm.step9.F90:349 and
  * SRC/do_somethingGPU_overlap.F90:48-58); with two steps per pass the

@@ -155,3 +155,34 @@ def test_driver_two_step_pairs_match_single_steps(gpu, tmp_path):
         assert np.array_equal(dg[row, 1:], [d[1], d[2], d[3], d[0], d[4]]), (it, dg[row], d)
         assert ds[row, 2] == s[1] and ds[row, 3] == s[0], (it, ds[row], s)
     assert dg.shape[0] == itfin // icheck
+
+
+@pytest.mark.gpu
+def test_driver_refcompat_two_step_pairs(gpu, tmp_path):
+    """The reference-compatible diagnostics (stale halo images) with the
+    driver's step pairs on a 256^3 lattice and an odd icheck: equal to the
+    reference call order (boundaries + col_MC every step) exactly."""
+    n, omega, itfin, icheck = 256, 1.95, 21, 7
+    svisc = (2.0 / omega - 1.0) / 6.0
+    deck = (f"&parameters\nlx={n}\nly={n}\nlz={n}\nomega={omega}\nitfin={itfin}\n"
+            f"icheck={icheck}\ninit='hit'\nk0=4.0\nurms=0.02\nseed=99\n"
+            f"storage='f32'\ndiag='refcompat'\n/\n")
+    print("[gpu-case] fortran driver 256^3 f32 refcompat pairs", flush=True)
+    run_driver(tmp_path, deck)
+    dg = np.loadtxt(tmp_path / "diagno.full.dat", ndmin=2)
+    ds = np.loadtxt(tmp_path / "dissipation.full.dat", ndmin=2)
+    lat = bgk.Lattice(n, n, n, omega=omega, storage="f32", refcompat_halos=True,
+                      two_step=False)
+    lat.init_hit(4.0, 0.02, 99)
+    row = 0
+    for it in range(1, itfin + 1):
+        lat.boundaries()
+        lat.col_MC()
+        if it % icheck == 0:
+            d = lat.diagno(refcompat=True)
+            s = lat.dissipation(svisc, refcompat=True)
+            assert dg[row, 0] == it and ds[row, 0] == it
+            assert np.array_equal(dg[row, 1:], [d[1], d[2], d[3], d[0], d[4]]), (it, dg[row], d)
+            assert ds[row, 2] == s[1] and ds[row, 3] == s[0], (it, ds[row], s)
+            row += 1
+    assert row == itfin // icheck
