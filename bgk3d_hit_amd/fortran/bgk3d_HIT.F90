@@ -267,8 +267,9 @@ subroutine col_MC(itime)
   if (itime < 0) write(6, *) itime
 end subroutine col_MC
 
-! Steps itime and itime+1 (boundaries of itime+1 included: a no-op on the
-! periodic lattice, recorded per step inside the library with refcompat halos)
+! Steps itime and itime+1 (boundaries of itime+1 included: nothing to exchange
+! on the periodic lattice; with refcompat halos the library keeps the halo
+! images of both steps itself)
 subroutine col_MC2(itime)
   use bgk_host
   implicit none
