@@ -364,6 +364,11 @@ int bgk_create(int lx, int ly, int lz, int storage, int layout, int flags, doubl
     bgk_ctx *c = new bgk_ctx();
     (void)hipGetDevice(&c->device);
     Lattice &L = c->lat;
+    L.ncu = 0;
+    if (hipDeviceGetAttribute(&L.ncu, hipDeviceAttributeMultiprocessorCount, c->device) !=
+            hipSuccess ||
+        L.ncu < 1)
+        L.ncu = 256;
     L.nx = lx;
     L.ny = ly;
     L.nz = lz;

@@ -191,21 +191,35 @@ struct Tile2 {
     static constexpr int X = wide ? 128 : 64, Y = 5, ZC = 64, MINB = 1;
 };
 
-// Planes per workgroup of the two-step pass: ZC, shortened on small lattices
-// so that the launch still has about 1024 workgroups; 0 when that would take
-// chunks of fewer than 16 planes (the two extra first-step planes per chunk
-// would cost more than the pass saves: take single steps).
-constexpr int TWO_STEP_MIN_BLOCKS = 1024, TWO_STEP_MIN_ZC = 16;
-// L.two_step == 2 takes chunks of at least min(16, nz) planes at any size.
+// Planes per workgroup of the two-step pass.  A workgroup fills a CU, so the
+// pass takes ceil(workgroups / CUs) rounds of (zc + 2) plane iterations; zc
+// in [8, ZC] minimises that (ties: the longer chunk, fewer redundant first-
+// step planes).  Measured at 128^3-512^3 the model picks the fastest chunk
+// of the sweep (e.g. 16 planes at 128^3 FP32: 208 workgroups, one round;
+// 13 planes would take 260, two rounds, and lose the gain).  0 (take single
+// steps) when the launch would not reach half of the CUs.
+constexpr int TWO_STEP_MIN_ZC = 8;
 // nz: the number of second-step planes of the pass (the slab's bulk passes
-// cover nz - 4 of its planes).
+// cover nz - 4 of its planes).  L.two_step == 2 fuses at any size.
 inline int two_step_zc(const Lattice &L, int tx, int ty, int zc_max, int nz) {
+    if (nz <= 0) return 0;
     const long long txy = (long long)((L.nx + tx - 1) / tx) * ((L.ny + ty - 1) / ty);
-    long long zc = txy * nz / TWO_STEP_MIN_BLOCKS;
-    if (L.two_step == 2 && zc < TWO_STEP_MIN_ZC) zc = TWO_STEP_MIN_ZC;
-    if (zc > zc_max) zc = zc_max;
-    if (zc > nz) zc = nz;
-    return zc < TWO_STEP_MIN_ZC && zc < nz ? 0 : (int)zc;
+    const long long ncu = L.ncu > 0 ? L.ncu : 1;
+    const int lo = nz < TWO_STEP_MIN_ZC ? nz : TWO_STEP_MIN_ZC;
+    const int hi = nz < zc_max ? nz : zc_max;
+    int best = lo;
+    long long best_cost = -1;
+    for (int zc = lo; zc <= hi; zc++) {
+        const long long nb = txy * ((nz + zc - 1) / zc);
+        const long long cost = ((nb + ncu - 1) / ncu) * (zc + 2);
+        if (best_cost < 0 || cost <= best_cost) {
+            best_cost = cost;
+            best = zc;
+        }
+    }
+    const long long nb = txy * ((nz + best - 1) / best);
+    if (L.two_step != 2 && 2 * nb < ncu) return 0;
+    return best;
 }
 inline int two_step_zc(const Lattice &L, int tx, int ty, int zc_max) {
     return two_step_zc(L, tx, ty, zc_max, L.layout == BGK_LAYOUT_PERIODIC ? L.nz : L.nz - 4);

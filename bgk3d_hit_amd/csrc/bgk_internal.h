@@ -36,6 +36,7 @@ struct Lattice {
                            // place) unless two_step, then one per copy
     int two_step;          // bgk_step may fuse two time steps per pass: 0 no,
                            // 1 where it pays, 2 at any size
+    int ncu;               // compute units of the device
 };
 
 struct Timing {
