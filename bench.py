@@ -13,12 +13,17 @@ the first step's state kept in LDS), which bgk_step takes on lattices this
 size; the one-pass-per-step kernel (k_collide) is reported beside it.
 
 `roofline` is measured live: HIP events around every collide launch on the
-stream it runs on give the average launch duration; the algorithmic bytes
-per launch are 19 loads + 19 stores of every site (38 * s * 512^3): one
-launch reads and writes the whole lattice once, for one time step
-(k_collide) or two (k_collide2).  `traffic` is the HBM bytes per launch
-measured by rocprofv3 FETCH_SIZE / WRITE_SIZE runs of this script
-(profiles/collide_traffic.json), null when absent.
+stream it runs on give the average launch duration.  The unit of work is the
+site-update, whose algorithmic bytes are 19 loads + 19 stores (38 * s); a
+launch processes 512^3 site-updates (k_collide, one time step) or 2 * 512^3
+(k_collide2, two time steps), so `achieved` = 38 * s * 512^3 * steps per
+launch / launch time.  k_collide2 moves each population through HBM once per
+two time steps, so its `achieved` can exceed the HBM peak: `frac` > 1 is the
+factor by which it beats the bandwidth bound of any one-step-per-launch
+kernel.  `traffic` is the HBM bytes per launch measured by rocprofv3
+FETCH_SIZE / WRITE_SIZE runs of this script (profiles/collide_traffic.json,
+null when absent or for another lattice size); `hbm_actual_gbs` = traffic /
+launch time is the kernel's own DRAM rate.
 
 `cpu_baseline` times the CPU oracle (plain C/OpenMP restatement of the
 reference Fortran) on a bounded sample on the host cores.
@@ -80,9 +85,9 @@ def run_gpu(kind, n, steps, warmup, diag_every, omega=1.95, shape=None, layout="
     avg_ms = t["collide_ms"] / launches
     step_collide_ms = t["collide_ms"] / steps
     two = per_step < 0.75
-    # HBM bytes a pass must move: every population read once and written once
-    # (one launch = one step, or two steps for the two-step pass)
-    algo_bytes = 38 * BYTES[kind] * nsite
+    # algorithmic bytes of the site-updates one launch processes: 38 * s per
+    # site-update, nsite per time step, one or two time steps per launch
+    algo_bytes = 38 * BYTES[kind] * nsite * (2 if two else 1)
     pass_ms = step_collide_ms * (2 if two else 1)
     achieved = algo_bytes / (pass_ms * 1e-3) / 1e9
     return {
@@ -101,6 +106,8 @@ def run_gpu(kind, n, steps, warmup, diag_every, omega=1.95, shape=None, layout="
         "achieved_gbs": achieved,
         "frac": achieved / HBM_PEAK_GBS,
         "algo_bytes_per_launch": algo_bytes,
+        "pass_ms": pass_ms,
+        "nsite": nsite,
         "ke0": d0[4],
         "diag_last": diag[-1] if diag else None,
     }
@@ -112,7 +119,9 @@ def summary(r, ref=None):
            "kernel": r["kernel"], "steps_per_launch": r["steps_per_launch"],
            "achieved_gbs": round(r["achieved_gbs"], 1), "frac": round(r["frac"], 4),
            "storage_bytes": BYTES[r["kind"]], "compute": COMPUTE[r["kind"]],
-           "traffic": load_traffic(r["kind"], r["two_step"])}
+           "traffic": load_traffic(r["kind"], r["two_step"], r["nsite"])}
+    if out["traffic"]:
+        out["hbm_actual_gbs"] = round(out["traffic"] / (r["collide_avg_ms"] * 1e-3) / 1e9, 1)
     if ref is not None:
         out["speedup_vs_f32"] = round(r["mlups"] / ref["mlups"], 3)
     return out
@@ -258,7 +267,10 @@ def calibrate_copy(gib, reps=10):
                       "avg_ms": ms, "gbs": nbytes / (ms * 1e-3) / 1e9}), flush=True)
 
 
-def load_traffic(kind, two_step):
+def load_traffic(kind, two_step, nsite):
+    """HBM bytes per launch of the collide kernel at 512^3 (rocprofv3 counters)."""
+    if nsite != 512 ** 3:
+        return None
     p = os.path.join(ROOT, "profiles", "collide_traffic.json")
     try:
         with open(p) as f:
@@ -267,6 +279,22 @@ def load_traffic(kind, two_step):
         return d.get(key, {}).get("hbm_bytes_per_launch")
     except (OSError, ValueError):
         return None
+
+
+def roofline(r):
+    traffic = load_traffic(r["kind"], r["two_step"], r["nsite"])
+    out = {"bound": "hbm", "achieved": round(r["achieved_gbs"], 1),
+           "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(r["frac"], 4),
+           "traffic": traffic, "kernel": r["kernel"],
+           "steps_per_launch": r["steps_per_launch"],
+           "avg_launch_ms": round(r["collide_avg_ms"], 4),
+           "algo_bytes_per_launch": r["algo_bytes_per_launch"],
+           "hbm_actual_gbs": None, "hbm_actual_frac": None}
+    if traffic:
+        act = traffic / (r["collide_avg_ms"] * 1e-3) / 1e9
+        out["hbm_actual_gbs"] = round(act, 1)
+        out["hbm_actual_frac"] = round(act / HBM_PEAK_GBS, 4)
+    return out
 
 
 def main():
@@ -352,22 +380,7 @@ def main():
                                f"and collide, diagno+dissipation every {a.diag_every} steps",
                    "lattice": [a.size] * 3, "omega": 1.95, "diag_every": a.diag_every,
                    "layout": "SoA, halo-free, periodic wrap in index math"},
-        "roofline": {"bound": "hbm", "achieved": round(main_run["achieved_gbs"], 1),
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(main_run["frac"], 4),
-                     "traffic": load_traffic("f32", main_run["two_step"]),
-                     "kernel": main_run["kernel"],
-                     "steps_per_launch": main_run["steps_per_launch"],
-                     "avg_launch_ms": round(main_run["collide_avg_ms"], 4),
-                     "algo_bytes_per_launch": main_run["algo_bytes_per_launch"],
-                     # 38*s bytes per site-update at the measured kernel rate: what a
-                     # one-step-per-pass kernel would have to stream to match it
-                     "one_step_equivalent_gbs": round(
-                         38 * BYTES["f32"] * a.size ** 3 / (main_run["collide_ms_per_step"]
-                                                          * 1e-3) / 1e9, 1),
-                     "one_step_equivalent_frac": round(
-                         38 * BYTES["f32"] * a.size ** 3 / (main_run["collide_ms_per_step"]
-                                                          * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
+        "roofline": roofline(main_run),
         "cpu_baseline": cpu,
         "cpu_samples": cpu_more,
         "variants": variants,

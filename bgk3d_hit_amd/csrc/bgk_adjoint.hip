@@ -250,8 +250,8 @@ hipError_t launch_tape2_o(bgk_ctx *c, C *tape0, C *tape1) {
     using T = Tile2<S, C, O32>;
     int nblocks;
     const Geo2 g = make_geo2(L, T::X, T::Y, two_step_zc(L, T::X, T::Y, T::ZC), &nblocks);
-    const CollideOp<C, true> op{(C)c->omega, (C)1, tape0, tape1, (size_t)L.nsite};
-    hipLaunchKernelGGL((k_collide2<S, C, O32, T::X, T::Y, T::MINB, true>), dim3(nblocks),
+    const CollideOp<C, true, true> op{(C)c->omega, (C)1, tape0, tape1, (size_t)L.nsite};
+    hipLaunchKernelGGL((k_collide2<S, C, O32, T::X, T::Y, T::MINB, true, false, true>), dim3(nblocks),
                        dim3(collide2_threads(T::X, T::Y)), 0, c->s0, pops_of<S>(L, c->cur),
                        pops_of<S>(L, 1 - c->cur), g, op);
     c->cur ^= 1;

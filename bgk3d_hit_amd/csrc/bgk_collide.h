@@ -299,8 +299,9 @@ __device__ __forceinline__ void lds_barrier() {
 // chunk and tile), `second` at the workgroup's own sites.  Extra holds inputs
 // loaded with a plane's populations (prefetched one plane ahead).
 
-// stream-collide; TAPE: record u = j/rho of both steps (slices tape0, tape1)
-template <typename C, bool TAPE>
+// stream-collide; TAPE: record u = j/rho of both steps (slices tape0, tape1);
+// SH: shifted storage only (collide_site<C, true>, cte1 = 1)
+template <typename C, bool TAPE, bool SH = false>
 struct CollideOp {
     static constexpr int SGN = 1;
     struct Extra {};
@@ -312,7 +313,7 @@ struct CollideOp {
     __device__ __forceinline__ void first(const C (&f)[19], C (&b)[19], const Extra &, size_t i,
                                           bool own) const {
         C ux, uy, uz;
-        collide_site<C>(f, b, omega, cte1, ux, uy, uz);
+        collide_site<C, SH>(f, b, omega, cte1, ux, uy, uz);
         if (TAPE && own) {
             tape0[i] = ux;
             tape0[ld + i] = uy;
@@ -322,7 +323,7 @@ struct CollideOp {
     __device__ __forceinline__ void second(const C (&f)[19], C (&b)[19], const Extra &,
                                            size_t i) const {
         C ux, uy, uz;
-        collide_site<C>(f, b, omega, cte1, ux, uy, uz);
+        collide_site<C, SH>(f, b, omega, cte1, ux, uy, uz);
         if (TAPE) {
             tape1[i] = ux;
             tape1[ld + i] = uy;
@@ -418,9 +419,9 @@ __device__ __forceinline__ void two_step_pass(const Pops<S> &src, const Pops<S> 
 #pragma unroll
         for (int q = 0; q < 19; q++) {
             f[q] = Storage<S>::template load<C>(cur[q]);
-            // a copy of its own: the next loads can then land in cur's
-            // registers, which carry them around the loop without a copy
-            // (a copy at the loop end would wait for the loads)
+            // a copy of its own: when cur and nxt are one register set, the
+            // next loads land in cur's registers, which carry them around the
+            // loop without a copy (a copy at the loop end would wait for them)
             asm volatile("" : "+v"(f[q]));
         }
         const X_t x1 = xcur;
@@ -497,15 +498,30 @@ __device__ __forceinline__ void two_step_pass(const Pops<S> &src, const Pops<S> 
     S ra[19];
     X_t xa;
     load_plane(za - 1, ra, xa);
-    for (int it = 0; it < nit; it++) iteration(it, ra, xa, ra, xa);
+    if constexpr (O32 && !FACES) {
+        // two register sets taking turns: the loads of plane z+1 land in the
+        // set plane z-1 used, so the compute copy of a plane costs no moves
+        // (not for the 64-bit-offset and face-writing variants, where the
+        // second set would spill)
+        S rb[19];
+        X_t xb;
+        int it = 0;
+        for (; it + 1 < nit; it += 2) {
+            iteration(it, ra, xa, rb, xb);
+            iteration(it + 1, rb, xb, ra, xa);
+        }
+        if (it < nit) iteration(it, ra, xa, rb, xb);
+    } else {
+        for (int it = 0; it < nit; it++) iteration(it, ra, xa, ra, xa);
+    }
 }
 
 template <typename S, typename C, bool O32, int TX, int TY, int MINB, bool TAPE,
-          bool FACES = false>
+          bool FACES = false, bool SH = false>
 __global__ __launch_bounds__(collide2_threads(TX, TY), MINB) void k_collide2(
-    Pops<S> src, Pops<S> dst, Geo2 g, CollideOp<C, TAPE> op) {
+    Pops<S> src, Pops<S> dst, Geo2 g, CollideOp<C, TAPE, SH> op) {
     __shared__ S lds[20][TY + 2][TX + 2];
-    two_step_pass<S, C, O32, TX, TY, CollideOp<C, TAPE>, FACES>(src, dst, g, op, lds);
+    two_step_pass<S, C, O32, TX, TY, CollideOp<C, TAPE, SH>, FACES>(src, dst, g, op, lds);
 }
 
 }  // namespace bgk

@@ -39,25 +39,31 @@ static hipError_t collide_t(const Lattice &L, int src, const Box &box, double om
     return hipGetLastError();
 }
 
-template <typename S, typename C, bool O32>
-static hipError_t collide2_o(const Lattice &L, int src, double omega, void *faces,
-                             hipStream_t s) {
+template <typename S, typename C, bool O32, bool SH>
+static hipError_t collide2_sh(const Lattice &L, int src, double omega, void *faces,
+                              hipStream_t s) {
     using T = Tile2<S, C, O32>;
     const int zc = two_step_zc(L, T::X, T::Y, T::ZC);
     if (zc <= 0) return hipErrorInvalidValue;  // caller checks two_step_ok first
     int nblocks;
     Geo2 g = make_geo2(L, T::X, T::Y, zc, &nblocks);
     g.faces = faces;
-    const CollideOp<C, false> op{(C)omega, (C)(L.shifted ? 1.0 : 0.0), nullptr, nullptr, 0};
+    const CollideOp<C, false, SH> op{(C)omega, (C)(L.shifted ? 1.0 : 0.0), nullptr, nullptr, 0};
     if (faces)
-        hipLaunchKernelGGL((k_collide2<S, C, O32, T::X, T::Y, T::MINB, false, true>),
+        hipLaunchKernelGGL((k_collide2<S, C, O32, T::X, T::Y, T::MINB, false, true, SH>),
                            dim3(nblocks), dim3(collide2_threads(T::X, T::Y)), 0, s,
                            pops_of<S>(L, src), pops_of<S>(L, 1 - src), g, op);
     else
-        hipLaunchKernelGGL((k_collide2<S, C, O32, T::X, T::Y, T::MINB, false>), dim3(nblocks),
-                           dim3(collide2_threads(T::X, T::Y)), 0, s, pops_of<S>(L, src),
-                           pops_of<S>(L, 1 - src), g, op);
+        hipLaunchKernelGGL((k_collide2<S, C, O32, T::X, T::Y, T::MINB, false, false, SH>),
+                           dim3(nblocks), dim3(collide2_threads(T::X, T::Y)), 0, s,
+                           pops_of<S>(L, src), pops_of<S>(L, 1 - src), g, op);
     return hipGetLastError();
+}
+template <typename S, typename C, bool O32>
+static hipError_t collide2_o(const Lattice &L, int src, double omega, void *faces,
+                             hipStream_t s) {
+    return L.shifted ? collide2_sh<S, C, O32, true>(L, src, omega, faces, s)
+                     : collide2_sh<S, C, O32, false>(L, src, omega, faces, s);
 }
 // two steps from copy `src` into copy 1-src (periodic layout, separate rest
 // arrays per copy)

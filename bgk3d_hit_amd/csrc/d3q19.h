@@ -114,16 +114,36 @@ struct K {
     static constexpr C p2 = (C)(1.0 / (3.0 * 3.0 * (1.0 + 3.0)));
 };
 
+// cte0 + y for the quadratic terms of collide_site.  SH (shifted storage,
+// cte1 = 1, cte0 = +0): +0 + y equals y bit for bit unless y is -0, and no y
+// passed here is -0: y = qf * (a - vsq) with a = (tre * v) * v or tre * v*v,
+// a product of two factors of equal sign (>= +0, never -0), vsq a sum of
+// squares (>= +0), so a - vsq is never -0 (round to nearest gives -0 only
+// for -0 - +0), and qf = 1.5 keeps the sign of a nonzero value without
+// underflowing to zero.  The add is then dropped (9 per site).
+template <bool SH, typename C>
+__device__ __forceinline__ C add_cte0(C cte0, C y) {
+    if constexpr (SH)
+        return y;
+    else
+        return cte0 + y;
+}
+
 // One site of SRC/col_MC.F90:133-219,333-351.  x[] are the pulled
 // populations, b[] receives the post-collision populations (b[18] is the
 // rest population the reference updates in place).  cte1 = 1 for shifted
 // storage (SRC/col_MC.F90:86-91).
 // ux, uy, uz receive the pulled-moment velocity j/rho (before the
 // reference scales it by rf for the linear terms).
-template <typename C>
-__device__ __forceinline__ void collide_site(const C (&x)[19], C (&b)[19], C omega, C cte1, C &ux,
-                                             C &uy, C &uz) {
+// SH: compiled for shifted storage only (cte1 = 1 whatever is passed): the
+// products cte1 * (...) fold to their second factor and add_cte0 drops the
+// additions of cte0 = +0; results are bit-identical to SH = false with
+// cte1 = 1.
+template <typename C, bool SH = false>
+__device__ __forceinline__ void collide_site(const C (&x)[19], C (&b)[19], C omega, C cte1_arg,
+                                             C &ux, C &uy, C &uz) {
     using k = K<C>;
+    const C cte1 = SH ? k::uno : cte1_arg;
     const C cte0 = k::uno - cte1;
     const C x01 = x[0], x02 = x[1], x03 = x[2], x04 = x[3], x05 = x[4], x06 = x[5], x07 = x[6];
     const C x08 = x[7], x09 = x[8], x10 = x[9], x11 = x[10], x12 = x[11], x13 = x[12];
@@ -142,21 +162,21 @@ __device__ __forceinline__ void collide_site(const C (&x)[19], C (&b)[19], C ome
     const C vx2 = vx * vx, vy2 = vy * vy, vz2 = vz * vz;
     const C vsq = vx2 + vy2 + vz2;
     C vxpy = vx + vy;
-    const C qxpy = cte0 + k::qf * (k::tre * vxpy * vxpy - vsq);
+    const C qxpy = add_cte0<SH>(cte0, k::qf * (k::tre * vxpy * vxpy - vsq));
     C vxmy = vx - vy;
-    const C qxmy = cte0 + k::qf * (k::tre * vxmy * vxmy - vsq);
+    const C qxmy = add_cte0<SH>(cte0, k::qf * (k::tre * vxmy * vxmy - vsq));
     C vxpz = vx + vz;
-    const C qxpz = cte0 + k::qf * (k::tre * vxpz * vxpz - vsq);
+    const C qxpz = add_cte0<SH>(cte0, k::qf * (k::tre * vxpz * vxpz - vsq));
     C vxmz = vx - vz;
-    const C qxmz = cte0 + k::qf * (k::tre * vxmz * vxmz - vsq);
+    const C qxmz = add_cte0<SH>(cte0, k::qf * (k::tre * vxmz * vxmz - vsq));
     C vypz = vy + vz;
-    const C qypz = cte0 + k::qf * (k::tre * vypz * vypz - vsq);
+    const C qypz = add_cte0<SH>(cte0, k::qf * (k::tre * vypz * vypz - vsq));
     C vymz = vy - vz;
-    const C qymz = cte0 + k::qf * (k::tre * vymz * vymz - vsq);
-    const C qx = cte0 + k::qf * (k::tre * vx2 - vsq);
-    const C qy = cte0 + k::qf * (k::tre * vy2 - vsq);
-    const C qz = cte0 + k::qf * (k::tre * vz2 - vsq);
-    const C q0 = cte0 + k::qf * (-vsq);
+    const C qymz = add_cte0<SH>(cte0, k::qf * (k::tre * vymz * vymz - vsq));
+    const C qx = add_cte0<SH>(cte0, k::qf * (k::tre * vx2 - vsq));
+    const C qy = add_cte0<SH>(cte0, k::qf * (k::tre * vy2 - vsq));
+    const C qz = add_cte0<SH>(cte0, k::qf * (k::tre * vz2 - vsq));
+    const C q0 = cte0 + k::qf * (-vsq);  // qf * (-vsq) is -0 when vsq = 0: kept
 
     vx = k::rf * vx;
     vy = k::rf * vy;
