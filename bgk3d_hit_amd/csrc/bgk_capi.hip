@@ -379,6 +379,8 @@ int bgk_create(int lx, int ly, int lz, int storage, int layout, int flags, doubl
     L.shifted = (flags & BGK_FLAG_NOSHIFT) ? 0 : 1;
     L.nsite = (long long)lx * ly * lz;
     L.plane_elems = plane;
+    L.ystride = lx;
+    L.zstride = (long long)lx * ly;
     c->flags = flags;
     c->omega = omega;
     c->timing.on = (flags & BGK_FLAG_TIMING) != 0;

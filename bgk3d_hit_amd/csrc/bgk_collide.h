@@ -46,10 +46,13 @@ struct Tile<bf16_t> {
 };
 
 // Launch geometry of a box of sites in TX x TY tiles (one z plane deep).
+// Population q of stored site (x, y, zs) is element x + y ystride + zs zstride
+// of its population pointer.
 struct Geo {
     int nx, ny, nz, nxy, zoff, periodic_z;
     int tiles_x, tiles_y;
     int x0, x1, y0, y1, z0;
+    long long ystride, zstride;
 };
 
 template <typename S>
@@ -68,6 +71,8 @@ static Geo make_geo(const Lattice &L, const Box &b, int *ntiles) {
     g.y0 = b.y0;
     g.y1 = b.y1;
     g.z0 = b.z0;
+    g.ystride = L.ystride;
+    g.zstride = L.zstride;
     *ntiles = g.tiles_x * g.tiles_y * (b.z1 - b.z0);
     return g;
 }
@@ -81,11 +86,12 @@ __device__ __forceinline__ int xcd_remap(int b, int G) {
     return (xcd < r) ? xcd * (q + 1) + slot : r * (q + 1) + (xcd - r) * q + slot;
 }
 
-// Element access by byte offset from a per-population base pointer.  With
-// O32 (every population array spans at most 4 GiB) the offset is a 32-bit
-// unsigned value: the base stays in scalar registers and each load or store
-// takes one vector offset register (global_load saddr form) instead of a
-// 64-bit address computed per population.
+// Element access by byte offset from a per-population base pointer.  The
+// kernels add a site's z plane to the base (uniform across the workgroup, so
+// the sum stays in scalar registers) and pass the in-plane part as the
+// offset.  With O32 (the in-plane offsets fit 32 bits) the offset is a
+// 32-bit unsigned value: each load or store takes one vector offset register
+// (global_load saddr form) instead of a 64-bit address per population.
 template <bool O32>
 using off_t_ = typename std::conditional<O32, uint32_t, size_t>::type;
 
@@ -97,10 +103,53 @@ template <typename S, typename O>
 __device__ __forceinline__ void st_at(S *base, O off, S v) {
     *(S *)((char *)base + off) = v;
 }
+// the same through a global-address-space base (plane_ptr); the element
+// moves as an unsigned integer of its size
+template <int N>
+struct uint_of;
+template <>
+struct uint_of<2> {
+    using type = uint16_t;
+};
+template <>
+struct uint_of<4> {
+    using type = uint32_t;
+};
+template <>
+struct uint_of<8> {
+    using type = uint64_t;
+};
+template <typename S, typename O>
+__device__ __forceinline__ S ld_at(__attribute__((address_space(1))) S *base, O off) {
+    using R = typename uint_of<sizeof(S)>::type;
+    const R r = *(__attribute__((address_space(1))) const R *)(
+        (__attribute__((address_space(1))) const char *)base + off);
+    return __builtin_bit_cast(S, r);
+}
+template <typename S, typename O>
+__device__ __forceinline__ void st_at(__attribute__((address_space(1))) S *base, O off, S v) {
+    using R = typename uint_of<sizeof(S)>::type;
+    *(__attribute__((address_space(1))) R *)((__attribute__((address_space(1))) char *)base +
+                                             off) = __builtin_bit_cast(R, v);
+}
 
-// whether byte offsets into one population array fit 32 bits
+// p + e for an element offset e that is uniform across the workgroup (a z
+// plane), kept in scalar registers: the asm makes the sum opaque, so the
+// compiler cannot fold a lane's in-plane offset into it (which would give
+// every access a 64-bit vector address).
+template <typename S>
+using gptr_t = __attribute__((address_space(1))) S *;
+template <typename S>
+__device__ __forceinline__ gptr_t<S> plane_ptr(S *p, size_t e) {
+    gptr_t<S> r = (gptr_t<S>)(p + e);
+    asm volatile("" : "+s"(r));
+    return r;
+}
+
+// whether the in-plane byte offsets (x + y ystride, plus the population
+// offsets of the slab plane buffer's pointers) fit 32 bits
 inline bool offsets_fit_32(const Lattice &L, size_t elem_bytes) {
-    return (unsigned long long)L.plane_elems * elem_bytes <= (1ULL << 32);
+    return (unsigned long long)L.zstride * elem_bytes <= (1ULL << 32);
 }
 
 // ------------------------------------------------------------------ collide
@@ -135,30 +184,33 @@ __global__ __launch_bounds__(Tile<S>::X * Tile<S>::Y) void k_collide(Pops<S> src
         zm = zs - 1;
         zp = zs + 1;
     }
-    // byte offsets of the x, y and z neighbours
+    // in-plane byte offsets of the x and y neighbours; element offsets of the
+    // z planes (uniform)
     const O X[3] = {(O)xm * es, (O)x * es, (O)xp * es};
-    const O Y[3] = {(O)(ym * g.nx) * es, (O)(y * g.nx) * es, (O)(yp * g.nx) * es};
-    const O Z[3] = {(O)zm * (O)g.nxy * es, (O)zs * (O)g.nxy * es, (O)zp * (O)g.nxy * es};
+    const O Y[3] = {(O)ym * (O)g.ystride * es, (O)y * (O)g.ystride * es,
+                    (O)yp * (O)g.ystride * es};
+    const size_t Z[3] = {(size_t)zm * g.zstride, (size_t)zs * g.zstride, (size_t)zp * g.zstride};
 
     C f[19];
 #pragma unroll
     for (int q = 0; q < 19; q++) {
         // pull: population q arrives from x - c_q
-        const O off = X[1 - cx(q)] + Y[1 - cy(q)] + Z[1 - cz(q)];
-        f[q] = Storage<S>::template load<C>(ld_at(src.p[q], off));
+        const O off = X[1 - cx(q)] + Y[1 - cy(q)];
+        f[q] = Storage<S>::template load<C>(ld_at(plane_ptr(src.p[q], Z[1 - cz(q)]), off));
     }
     C b[19];
     C ux, uy, uz;
     collide_site<C>(f, b, omega, cte1, ux, uy, uz);
-    const O c = X[1] + Y[1] + Z[1];
-    if (TAPE) {
-        const size_t i = (size_t)(c / es);
+    const O c = X[1] + Y[1];
+    if (TAPE) {  // compact interior site index
+        const size_t i = (size_t)x + (size_t)g.nx * y + (size_t)g.nxy * z;
         tape[i] = ux;
         tape[tape_ld + i] = uy;
         tape[2 * tape_ld + i] = uz;
     }
 #pragma unroll
-    for (int q = 0; q < 19; q++) st_at(dst.p[q], c, Storage<S>::template store<C>(b[q]));
+    for (int q = 0; q < 19; q++)
+        st_at(plane_ptr(dst.p[q], Z[1]), c, Storage<S>::template store<C>(b[q]));
 }
 
 // --------------------------------------------------------- two-step collide
@@ -258,6 +310,7 @@ struct Geo2 {
     int nx, ny, nz, nxy;
     int tiles_x, tiles_y, zc;
     int zlo, zhi, zoff, periodic_z;
+    long long ystride, zstride;  // element pitches of rows and planes (as Geo)
     void *faces;  // FACES: face images of the first step (k_pack_faces layout)
 };
 
@@ -274,6 +327,8 @@ static Geo2 make_geo2(const Lattice &L, int tx, int ty, int zc, int *nblocks) {
     g.zoff = L.zoff;
     g.zlo = g.periodic_z ? 0 : 2;
     g.zhi = g.periodic_z ? L.nz : L.nz - 2;
+    g.ystride = L.ystride;
+    g.zstride = L.zstride;
     g.faces = nullptr;
     *nblocks = g.tiles_x * g.tiles_y * ((g.zhi - g.zlo + g.zc - 1) / g.zc);
     return g;
@@ -383,25 +438,37 @@ __device__ __forceinline__ void two_step_pass(const Pops<S> &src, const Pops<S> 
     const int xp = (gx == g.nx - 1) ? 0 : gx + 1;
     const int ym = (gy == 0) ? g.ny - 1 : gy - 1;
     const int yp = (gy == g.ny - 1) ? 0 : gy + 1;
-    // byte offsets of the x and y neighbours
+    // in-plane byte offsets of the x and y neighbours (z planes are added to
+    // the population pointers, uniform across the workgroup)
     const O X[3] = {(O)xm * es, (O)gx * es, (O)xp * es};
-    const O Y[3] = {(O)(ym * g.nx) * es, (O)(gy * g.nx) * es, (O)(yp * g.nx) * es};
-    const O plane = (O)g.nxy * es;
+    const O Y[3] = {(O)ym * (O)g.ystride * es, (O)gy * (O)g.ystride * es,
+                    (O)yp * (O)g.ystride * es};
+    const size_t plane = (size_t)g.zstride;
 
     // z in [zlo-1, zhi]; a slab's planes need no wrap
     auto wrapz = [&](int z) {
         return !g.periodic_z ? z : (z < 0 ? z + g.nz : (z >= g.nz ? z - g.nz : z));
     };
-    auto site_of = [&](int p) { return (size_t)((X[1] + Y[1] + (O)(p + g.zoff) * plane) / es); };
+    // compact index of the column's site at stored plane p + zoff (auxiliary
+    // arrays of the per-site operation: tape, adjoint inputs)
+    auto site_of = [&](int p) {
+        return (size_t)gx + (size_t)g.nx * gy + (size_t)g.nxy * (p + g.zoff);
+    };
     auto load_plane = [&](int z, S(&raw)[19], X_t &xr) {
         const int p = wrapz(z);
         const int zm = !g.periodic_z ? p - 1 : ((p == 0) ? g.nz - 1 : p - 1);
         const int zp = !g.periodic_z ? p + 1 : ((p == g.nz - 1) ? 0 : p + 1);
-        const O Z[3] = {(O)(zm + g.zoff) * plane, (O)(p + g.zoff) * plane,
-                        (O)(zp + g.zoff) * plane};
+        const size_t Z[3] = {(size_t)(zm + g.zoff) * plane, (size_t)(p + g.zoff) * plane,
+                             (size_t)(zp + g.zoff) * plane};
 #pragma unroll
-        for (int q = 0; q < 19; q++)
-            raw[q] = ld_at(src.p[q], X[1 - SGN * cx(q)] + Y[1 - SGN * cy(q)] + Z[1 - SGN * cz(q)]);
+        for (int q = 0; q < 19; q++) {
+            // opaque per plane, so that the compiler does not keep the 64-bit
+            // extension of the offset live across the loop (a 32-bit offset
+            // register is what the scalar-base load form takes)
+            O off = X[1 - SGN * cx(q)] + Y[1 - SGN * cy(q)];
+            asm volatile("" : "+v"(off));
+            raw[q] = ld_at(plane_ptr(src.p[q], Z[1 - SGN * cz(q)]), off);
+        }
         xr = op.extra_first(site_of(p));
     };
 
@@ -478,9 +545,12 @@ __device__ __forceinline__ void two_step_pass(const Pops<S> &src, const Pops<S> 
             }
             C b[19];
             op.second(f2, b, x2, site_of(p2));
-            const O c = X[1] + Y[1] + (O)(p2 + g.zoff) * plane;
+            O c = X[1] + Y[1];
+            asm volatile("" : "+v"(c));
+            const size_t zo = (size_t)(p2 + g.zoff) * plane;
 #pragma unroll
-            for (int q = 0; q < 19; q++) st_at(dst.p[q], c, Storage<S>::template store<C>(b[q]));
+            for (int q = 0; q < 19; q++)
+                st_at(plane_ptr(dst.p[q], zo), c, Storage<S>::template store<C>(b[q]));
         }
         lds_barrier();
         if (active) {

@@ -31,6 +31,8 @@ struct Lattice {
     int shifted;
     long long nsite;       // nx*ny*nz
     long long plane_elems; // nx*ny*nzs stored values per population
+    long long ystride;     // elements between rows y and y+1 of a population
+    long long zstride;     // elements between stored planes z and z+1
     void *pop[2][19];      // population base pointers of copy 0/1; the rest
                            // population [c][18] is one shared array (updated in
                            // place) unless two_step, then one per copy
