@@ -39,14 +39,16 @@ namespace bgk {
 // lambda^T from at-site moments of the final state, stored in the shifted
 // form M^T_i(x + c_i) = lambda^T_i(x)
 template <typename S, typename C>
-__global__ __launch_bounds__(256) void k_lambda_final(Pops<S> P, C *M, int nx, int ny, int nz) {
+__global__ __launch_bounds__(256) void k_lambda_final(Pops<S> P, SiteMap m, C *M, int nx, int ny,
+                                                      int nz) {
     const long long nsite = (long long)nx * ny * nz;
     const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
     if (i >= nsite) return;
     const int x = (int)(i % nx), y = (int)((i / nx) % ny), z = (int)(i / ((long long)nx * ny));
+    const long long o = site_elem(m, i);
     C f[19];
 #pragma unroll
-    for (int q = 0; q < 19; q++) f[q] = Storage<S>::template load<C>(P.p[q][i]);
+    for (int q = 0; q < 19; q++) f[q] = Storage<S>::template load<C>(P.p[q][o]);
     C rho, ux, uy, uz;
     moments_site<C>(f, (C)1, rho, ux, uy, uz);
     const C usq = ux * ux + uy * uy + uz * uz;
@@ -190,8 +192,7 @@ namespace {
 
 struct Adj {
     bgk_ctx *c;
-    size_t pbytes;      // bytes of one stored population plane
-    size_t state_bytes; // 19 planes
+    size_t state_bytes; // one copy of the populations (all 19, interleaved)
     size_t csize;       // sizeof compute kind
     std::vector<void *> allocs;
     size_t bytes = 0;
@@ -207,21 +208,18 @@ struct Adj {
         bytes += n;
         return BGK_OK;
     }
-    // copy the current state (copy c->cur) to / from a 19-plane slot
+    // copy the current state (copy c->cur, whose block starts at population
+    // 0's pointer) to / from a slot
     int save(void *slot) {
         const Lattice &L = c->lat;
-        BGK_HIP(hipMemcpyAsync(slot, L.pop[c->cur][0], pbytes * 18, hipMemcpyDeviceToDevice,
+        BGK_HIP(hipMemcpyAsync(slot, L.pop[c->cur][0], state_bytes, hipMemcpyDeviceToDevice,
                                c->s0));
-        BGK_HIP(hipMemcpyAsync((char *)slot + pbytes * 18, L.pop[c->cur][18], pbytes,
-                               hipMemcpyDeviceToDevice, c->s0));
         return BGK_OK;
     }
     int restore(const void *slot) {
         const Lattice &L = c->lat;
-        BGK_HIP(hipMemcpyAsync(L.pop[c->cur][0], slot, pbytes * 18, hipMemcpyDeviceToDevice,
+        BGK_HIP(hipMemcpyAsync(L.pop[c->cur][0], slot, state_bytes, hipMemcpyDeviceToDevice,
                                c->s0));
-        BGK_HIP(hipMemcpyAsync(L.pop[c->cur][18], (const char *)slot + pbytes * 18, pbytes,
-                               hipMemcpyDeviceToDevice, c->s0));
         return BGK_OK;
     }
 };
@@ -281,7 +279,9 @@ hipError_t launch_adjoint2_o(bgk_ctx *c, const C *lin, C *lout, const C *tape0,
     const Lattice &L = c->lat;
     using T = Tile2<C, C, O32>;
     int nblocks;
-    const Geo2 g = make_geo2(L, T::X, T::Y, two_step_zc(L, T::X, T::Y, T::ZC), &nblocks);
+    Geo2 g = make_geo2(L, T::X, T::Y, two_step_zc(L, T::X, T::Y, T::ZC), &nblocks);
+    g.ystride = L.nx;  // lambda: 19 separate arrays of nsite values
+    g.zstride = (long long)L.nx * L.ny;
     Pops<C> a, b;
     for (int q = 0; q < 19; q++) {
         a.p[q] = (C *)lin + (size_t)q * L.nsite;
@@ -341,8 +341,7 @@ int adjoint_t(bgk_ctx *c, const double *u0, int T, int K, int levels, double *J,
     const long long N = L.nsite;
     Adj a;
     a.c = c;
-    a.pbytes = ((size_t)L.plane_elems * sizeof(S) + 255) & ~(size_t)255;
-    a.state_bytes = a.pbytes * 19;
+    a.state_bytes = ((size_t)L.copy_elems * sizeof(S) + 255) & ~(size_t)255;
     a.csize = sizeof(C);
     if (K < 1) K = 1;
     if (levels == 0) K = std::max(T, 1);
@@ -382,7 +381,7 @@ int adjoint_t(bgk_ctx *c, const double *u0, int T, int K, int levels, double *J,
     }
     if ((r = reduce_ke(c, c->cur, J))) return r;  // synchronises
     hipLaunchKernelGGL((k_lambda_final<S, C>), dim3((unsigned)((N + 255) / 256)), dim3(256), 0,
-                       c->s0, pops_of<S>(L, c->cur), lamA, L.nx, L.ny, L.nz);
+                       c->s0, pops_of<S>(L, c->cur), site_map(L), lamA, L.nx, L.ny, L.nz);
     BGK_HIP(hipGetLastError());
     BGK_HIP(hipStreamSynchronize(c->s0));
     const double fwd_ms = ms_since(t0);

@@ -379,35 +379,32 @@ int bgk_create(int lx, int ly, int lz, int storage, int layout, int flags, doubl
     L.shifted = (flags & BGK_FLAG_NOSHIFT) ? 0 : 1;
     L.nsite = (long long)lx * ly * lz;
     L.plane_elems = plane;
-    L.ystride = lx;
-    L.zstride = (long long)lx * ly;
+    L.ystride = 19LL * lx;
+    L.zstride = 19LL * lx * ly;
+    L.copy_elems = 19LL * plane;
     c->flags = flags;
     c->omega = omega;
     c->timing.on = (flags & BGK_FLAG_TIMING) != 0;
 
     const size_t es = storage_bytes(storage);
-    const size_t pbytes = ((size_t)plane * es + 255) & ~(size_t)255;
+    const size_t cbytes = ((size_t)L.copy_elems * es + 255) & ~(size_t)255;
     int rc = BGK_OK;
     auto bail = [&](int code) {
         bgk_destroy(c);
         return code;
     };
-    // Two-step fusion reads the source copy's rest population at grown-tile
-    // sites that another workgroup writes, so it needs a rest array per copy.
     L.two_step = !(flags & BGK_FLAG_SINGLE_STEP)
                      ? ((flags & BGK_FLAG_TWO_STEP_ANY_SIZE) ? 2 : 1)
                      : 0;
-    const int nplanes = L.two_step ? 38 : 37;
-    hipError_t e = hipMalloc(&c->pop_alloc, pbytes * nplanes);
+    // two copies in the interleaved [z][y][q][x] layout (see Lattice)
+    hipError_t e = hipMalloc(&c->pop_alloc, cbytes * 2);
     if (e != hipSuccess) return bail(hip_fail(e, "hipMalloc(populations)"));
-    c->bytes += pbytes * nplanes;
+    c->bytes += cbytes * 2;
     char *base = (char *)c->pop_alloc;
-    for (int q = 0; q < 18; q++) {
-        L.pop[0][q] = base + pbytes * q;
-        L.pop[1][q] = base + pbytes * (18 + q);
+    for (int q = 0; q < 19; q++) {
+        L.pop[0][q] = base + (size_t)q * lx * es;
+        L.pop[1][q] = base + cbytes + (size_t)q * lx * es;
     }
-    L.pop[0][18] = base + pbytes * 36;
-    L.pop[1][18] = base + pbytes * (nplanes - 1);
 
     if ((e = hipStreamCreateWithFlags(&c->s0, hipStreamNonBlocking)) != hipSuccess)
         return bail(hip_fail(e, "hipStreamCreate"));
@@ -417,7 +414,7 @@ int bgk_create(int lx, int ly, int lz, int storage, int layout, int flags, doubl
         return bail(hip_fail(e, "hipEventCreate"));
     if ((e = hipEventCreateWithFlags(&c->ev1, hipEventDisableTiming)) != hipSuccess)
         return bail(hip_fail(e, "hipEventCreate"));
-    if ((e = hipMemsetAsync(c->pop_alloc, 0, pbytes * nplanes, c->s0)) != hipSuccess)
+    if ((e = hipMemsetAsync(c->pop_alloc, 0, cbytes * 2, c->s0)) != hipSuccess)
         return bail(hip_fail(e, "hipMemsetAsync(populations)"));
 
     if (layout == BGK_LAYOUT_SLAB_Z) {
@@ -425,13 +422,15 @@ int bgk_create(int lx, int ly, int lz, int storage, int layout, int flags, doubl
         if ((e = hipMalloc(&c->halo_send, c->halo_bytes)) != hipSuccess)
             return bail(hip_fail(e, "hipMalloc(halo)"));
         c->bytes += c->halo_bytes;
-        if (L.two_step) {  // first-step plane buffer of the two-step boundary path
-            const size_t pb8 = ((size_t)8 * lx * ly * es + 255) & ~(size_t)255;
-            if ((e = hipMalloc(&c->slab_s1, pb8 * 19)) != hipSuccess)
+        if (L.two_step) {  // first-step plane buffer of the two-step boundary path:
+                           // 8 planes in the lattice's layout (same pitches)
+            const size_t pb8 = ((size_t)8 * L.zstride * es + 255) & ~(size_t)255;
+            if ((e = hipMalloc(&c->slab_s1, pb8)) != hipSuccess)
                 return bail(hip_fail(e, "hipMalloc(slab plane buffer)"));
-            c->bytes += pb8 * 19;
-            for (int q = 0; q < 19; q++) c->slab_s1_pop[q] = (char *)c->slab_s1 + pb8 * q;
-            if ((e = hipMemsetAsync(c->slab_s1, 0, pb8 * 19, c->s0)) != hipSuccess)
+            c->bytes += pb8;
+            for (int q = 0; q < 19; q++)
+                c->slab_s1_pop[q] = (char *)c->slab_s1 + (size_t)q * lx * es;
+            if ((e = hipMemsetAsync(c->slab_s1, 0, pb8, c->s0)) != hipSuccess)
                 return bail(hip_fail(e, "hipMemsetAsync(slab plane buffer)"));
         }
     }

@@ -21,8 +21,17 @@ struct Box {
 };
 
 // Everything a kernel launch needs to find the populations.
-//   periodic layout: plane z of a population is at offset z*nx*ny
-//   slab layout    : interior plane z is at (z+1)*nx*ny, ghosts at 0, nz+1
+//
+// Layout in HBM: each of the two copies (the reference's a/b arrays) is one
+// block ordered [z][y][q][x]: stored plane z, row y, then the 19 populations
+// of that row side by side, x fastest.  Population q of stored site
+// (x, y, zs) is element x + y*ystride + zs*zstride of pop[copy][q], with
+// pop[copy][q] = copy base + q*nx, ystride = 19*nx, zstride = 19*nx*ny.  A
+// row of all 19 populations is one contiguous 19*nx*s byte run, so the 38
+// load and store streams of a step stay within a few DRAM pages (measured
+// +10 % on the collide kernels against 19 separate arrays per copy).
+//   periodic layout: interior plane z is stored plane z
+//   slab layout    : interior plane z is stored plane z+1, ghosts 0 and nz+1
 struct Lattice {
     int nx, ny, nz;  // interior sites
     int zoff;        // 0 periodic, 1 slab
@@ -30,16 +39,31 @@ struct Lattice {
     int layout, storage;
     int shifted;
     long long nsite;       // nx*ny*nz
-    long long plane_elems; // nx*ny*nzs stored values per population
+    long long plane_elems; // nx*ny*nzs stored sites
     long long ystride;     // elements between rows y and y+1 of a population
     long long zstride;     // elements between stored planes z and z+1
-    void *pop[2][19];      // population base pointers of copy 0/1; the rest
-                           // population [c][18] is one shared array (updated in
-                           // place) unless two_step, then one per copy
+    long long copy_elems;  // elements of one copy (19*nx*ny*nzs)
+    void *pop[2][19];      // population base pointers of copy 0/1
     int two_step;          // bgk_step may fuse two time steps per pass: 0 no,
                            // 1 where it pays, 2 at any size
     int ncu;               // compute units of the device
 };
+
+// Element offset of interior site i (x fastest, then y, then z) of a
+// population, for kernels that walk the sites in order.
+struct SiteMap {
+    int nx, ny, zoff;
+    long long ystride, zstride;
+};
+inline SiteMap site_map(const Lattice &L) {
+    return SiteMap{L.nx, L.ny, L.zoff, L.ystride, L.zstride};
+}
+__host__ __device__ inline long long site_elem(const SiteMap &m, long long i) {
+    const long long nxy = (long long)m.nx * m.ny;
+    const long long z = i / nxy, r = i - z * nxy;
+    const long long y = r / m.nx, x = r - y * m.nx;
+    return x + y * m.ystride + (z + m.zoff) * m.zstride;
+}
 
 struct Timing {
     bool on = false;
@@ -70,7 +94,7 @@ struct bgk_ctx {
     hipStream_t s1 = nullptr;  // halo / boundary stream (slab overlap)
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
 
-    void *pop_alloc = nullptr;  // one allocation holding 37 (38) population planes
+    void *pop_alloc = nullptr;  // one allocation holding the two copies
 
     // slab halo message: [face 0 = z+ (plane nz), 1 = z- (plane 1)][5][nx*ny]
     void *halo_send = nullptr;

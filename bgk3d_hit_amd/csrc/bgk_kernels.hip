@@ -123,6 +123,7 @@ __global__ __launch_bounds__(FIN_THREADS) void k_finalize(const double *partials
 struct RedGeo {
     int nx, ny, nz, nxy, zoff, periodic_z;
     int tiles_x, tiles_y, zb, nparts;
+    long long ystride, zstride;  // population pitches (Lattice)
 };
 
 template <typename S>
@@ -139,6 +140,8 @@ static RedGeo make_red_geo(const Lattice &L) {
     const long long txy = (long long)g.tiles_x * g.tiles_y;
     g.zb = (int)std::max<long long>(1, (txy * L.nz + RED_TARGET_PARTS - 1) / RED_TARGET_PARTS);
     g.nparts = (int)(txy * ((L.nz + g.zb - 1) / g.zb));
+    g.ystride = L.ystride;
+    g.zstride = L.zstride;
     return g;
 }
 
@@ -170,7 +173,7 @@ __global__ __launch_bounds__(Tile<S>::X * Tile<S>::Y) void k_diagno(Pops<S> P, R
     if (x < g.nx && y < g.ny) {
         const int z1 = min(g.nz, (zc + 1) * g.zb);
         for (int z = zc * g.zb; z < z1; z++) {
-            const size_t site = (size_t)x + (size_t)g.nx * y + (size_t)g.nxy * (z + g.zoff);
+            const size_t site = (size_t)x + (size_t)g.ystride * y + (size_t)g.zstride * (z + g.zoff);
             C f[19];
 #pragma unroll
             for (int q = 0; q < 19; q++) f[q] = Storage<S>::template load<C>(P.p[q][site]);
@@ -212,7 +215,8 @@ __global__ __launch_bounds__(Tile<S>::X * Tile<S>::Y) void k_dissipation(Pops<S>
         const int xm = (x == 0) ? nx - 1 : x - 1, xp = (x == nx - 1) ? 0 : x + 1;
         const int ym = (y == 0) ? ny - 1 : y - 1, yp = (y == ny - 1) ? 0 : y + 1;
         const int X[3] = {xm, x, xp};
-        const int Y[3] = {ym * nx, y * nx, yp * nx};
+        const int Yc[3] = {ym, y, yp};
+        const long long Y[3] = {ym * g.ystride, y * g.ystride, yp * g.ystride};
         const int z1 = min(nz, (zc + 1) * g.zb);
         for (int z = zc * g.zb; z < z1; z++) {
             const int zs = z + g.zoff;
@@ -224,7 +228,8 @@ __global__ __launch_bounds__(Tile<S>::X * Tile<S>::Y) void k_dissipation(Pops<S>
                 zm = zs - 1;
                 zp = zs + 1;
             }
-            const int Z[3] = {zm * nxy, zs * nxy, zp * nxy};
+            const int Zc[3] = {zm, zs, zp};
+            const long long Z[3] = {zm * g.zstride, zs * g.zstride, zp * g.zstride};
             C f[19];
 #pragma unroll
             for (int q = 0; q < 19; q++) {
@@ -236,8 +241,8 @@ __global__ __launch_bounds__(Tile<S>::X * Tile<S>::Y) void k_dissipation(Pops<S>
                     const bool ox = (sx < 0 || sx >= nx), oy = (sy < 0 || sy >= ny);
                     const bool oz = (sz < 0 || sz >= nz);
                     if (ox | oy | oz) {
-                        const int wx = X[1 - cx(q)], wy = (Y[1 - cy(q)]) / nx;
-                        const int wz = (Z[1 - cz(q)]) / nxy;
+                        const int wx = X[1 - cx(q)], wy = Yc[1 - cy(q)];
+                        const int wz = Zc[1 - cz(q)];
                         const S *gx = gcache;
                         const S *gy = gcache + (size_t)19 * ny * nz;
                         const S *gz = gy + (size_t)19 * nx * nz;
@@ -309,14 +314,15 @@ __device__ __forceinline__ unsigned long long raw_bits(S v) {
 
 template <typename S>
 __global__ __launch_bounds__(RED_THREADS) void k_checksum(Pops<S> P, long long nsite,
-                                                          long long off,
+                                                          SiteMap m,
                                                           unsigned long long *partials) {
     unsigned long long h = 0;
     for (long long i = (long long)blockIdx.x * RED_THREADS + threadIdx.x; i < nsite;
          i += (long long)gridDim.x * RED_THREADS) {
+        const long long o = site_elem(m, i);
 #pragma unroll
         for (int q = 0; q < 19; q++)
-            h += mix64(raw_bits(P.p[q][off + i]) ^ ((unsigned long long)q << 58) ^
+            h += mix64(raw_bits(P.p[q][o]) ^ ((unsigned long long)q << 58) ^
                        ((unsigned long long)i * 0xD1B54A32D192ED03ull));
     }
     for (int o = 32; o > 0; o >>= 1) h += __shfl_xor(h, o);
@@ -350,7 +356,7 @@ template <typename S>
 static hipError_t checksum_t(const Lattice &L, int cur, void *partials, int nblocks, void *out,
                              hipStream_t s) {
     hipLaunchKernelGGL((k_checksum<S>), dim3(nblocks), dim3(RED_THREADS), 0, s, pops_of<S>(L, cur),
-                       L.nsite, (long long)L.zoff * L.nx * L.ny, (unsigned long long *)partials);
+                       L.nsite, site_map(L), (unsigned long long *)partials);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_checksum_final, dim3(1), dim3(RED_THREADS), 0, s,
@@ -367,24 +373,28 @@ __constant__ int c_zminus[5] = {1, 8, 10, 14, 15}; // 02 09 11 15 16: leave thro
 // `top`: the top interior plane), face 1 = the -z message (stored plane
 // `bot`: the bottom interior plane)
 template <typename S>
-__global__ __launch_bounds__(256) void k_pack_z(Pops<S> P, int nxy, int top, int bot, S *send) {
+__global__ __launch_bounds__(256) void k_pack_z(Pops<S> P, int nx, int nxy, long long ystride,
+                                                long long zstride, int top, int bot, S *send) {
     const int p = blockIdx.x * 256 + threadIdx.x;
     if (p >= nxy) return;
     const int q5 = blockIdx.y;  // 0..4
-    send[(size_t)(0 * 5 + q5) * nxy + p] = P.p[c_zplus[q5]][(size_t)top * nxy + p];
-    send[(size_t)(1 * 5 + q5) * nxy + p] = P.p[c_zminus[q5]][(size_t)bot * nxy + p];
+    const size_t o = (size_t)(p % nx) + (size_t)(p / nx) * ystride;
+    send[(size_t)(0 * 5 + q5) * nxy + p] = P.p[c_zplus[q5]][(size_t)top * zstride + o];
+    send[(size_t)(1 * 5 + q5) * nxy + p] = P.p[c_zminus[q5]][(size_t)bot * zstride + o];
 }
 
 // the +z message fills the bottom ghost plane `lo`, the -z message the top
 // ghost plane `hi` (SRC/bcond.comm.step3.F90:97-106, 133-142)
 template <typename S>
-__global__ __launch_bounds__(256) void k_unpack_z(Pops<S> P, int nxy, int lo, int hi,
+__global__ __launch_bounds__(256) void k_unpack_z(Pops<S> P, int nx, int nxy, long long ystride,
+                                                  long long zstride, int lo, int hi,
                                                   const S *recv) {
     const int p = blockIdx.x * 256 + threadIdx.x;
     if (p >= nxy) return;
     const int q5 = blockIdx.y;
-    P.p[c_zplus[q5]][(size_t)lo * nxy + p] = recv[(size_t)(0 * 5 + q5) * nxy + p];
-    P.p[c_zminus[q5]][(size_t)hi * nxy + p] = recv[(size_t)(1 * 5 + q5) * nxy + p];
+    const size_t o = (size_t)(p % nx) + (size_t)(p / nx) * ystride;
+    P.p[c_zplus[q5]][(size_t)lo * zstride + o] = recv[(size_t)(0 * 5 + q5) * nxy + p];
+    P.p[c_zminus[q5]][(size_t)hi * zstride + o] = recv[(size_t)(1 * 5 + q5) * nxy + p];
 }
 
 template <typename S>
@@ -398,7 +408,7 @@ static hipError_t pack_planes_t(const Lattice &L, void *const *pops, int top, in
                                 void *send, hipStream_t s) {
     const int nxy = L.nx * L.ny;
     hipLaunchKernelGGL((k_pack_z<S>), dim3((nxy + 255) / 256, 5), dim3(256), 0, s,
-                       pops_at<S>(pops), nxy, top, bot, (S *)send);
+                       pops_at<S>(pops), L.nx, nxy, L.ystride, L.zstride, top, bot, (S *)send);
     return hipGetLastError();
 }
 template <typename S>
@@ -406,7 +416,8 @@ static hipError_t unpack_planes_t(const Lattice &L, void *const *pops, int lo, i
                                   const void *recv, hipStream_t s) {
     const int nxy = L.nx * L.ny;
     hipLaunchKernelGGL((k_unpack_z<S>), dim3((nxy + 255) / 256, 5), dim3(256), 0, s,
-                       pops_at<S>(pops), nxy, lo, hi, (const S *)recv);
+                       pops_at<S>(pops), L.nx, nxy, L.ystride, L.zstride, lo, hi,
+                       (const S *)recv);
     return hipGetLastError();
 }
 template <typename S>
@@ -431,11 +442,10 @@ static hipError_t collide_planes_t(const Lattice &L, void *const *src, int src_o
     Geo g = make_geo<S>(L, box, &ntiles);
     g.zoff = 0;
     g.periodic_z = 0;
-    const size_t nxy = (size_t)L.nx * L.ny;
     Pops<S> a, b;
     for (int q = 0; q < 19; q++) {
-        a.p[q] = (S *)src[q] + (ptrdiff_t)src_off * (ptrdiff_t)nxy;
-        b.p[q] = (S *)dst[q] + (ptrdiff_t)dst_off * (ptrdiff_t)nxy;
+        a.p[q] = (S *)src[q] + (ptrdiff_t)src_off * (ptrdiff_t)L.zstride;
+        b.p[q] = (S *)dst[q] + (ptrdiff_t)dst_off * (ptrdiff_t)L.zstride;
     }
     // 64-bit offsets: the shifted base pointers may lie below an array
     hipLaunchKernelGGL((k_collide<S, C, false>), dim3(ntiles), dim3(Tile<S>::X, Tile<S>::Y), 0, s,
@@ -446,7 +456,8 @@ static hipError_t collide_planes_t(const Lattice &L, void *const *src, int src_o
 // face images of the periodic layout: gx[q] = plane x = (c_x>0 ? nx-1 : 0)
 // of population q for the 10 populations with c_x != 0, likewise gy, gz.
 template <typename S>
-__global__ __launch_bounds__(256) void k_pack_faces(Pops<S> P, int nx, int ny, int nz, S *g) {
+__global__ __launch_bounds__(256) void k_pack_faces(Pops<S> P, int nx, int ny, int nz,
+                                                    long long ys, long long zs, S *g) {
     const int axis = blockIdx.y;
     const long long p = (long long)blockIdx.x * 256 + threadIdx.x;
     const int nxy = nx * ny;
@@ -460,7 +471,7 @@ __global__ __launch_bounds__(256) void k_pack_faces(Pops<S> P, int nx, int ny, i
         for (int q = 0; q < 18; q++) {
             if (cx(q) == 0) continue;
             const int x = cx(q) > 0 ? nx - 1 : 0;
-            gx[(size_t)q * ny * nz + p] = P.p[q][x + nx * y + (size_t)nxy * z];
+            gx[(size_t)q * ny * nz + p] = P.p[q][x + ys * y + zs * z];
         }
     } else if (axis == 1) {
         if (p >= (long long)nx * nz) return;
@@ -469,7 +480,7 @@ __global__ __launch_bounds__(256) void k_pack_faces(Pops<S> P, int nx, int ny, i
         for (int q = 0; q < 18; q++) {
             if (cy(q) == 0) continue;
             const int y = cy(q) > 0 ? ny - 1 : 0;
-            gy[(size_t)q * nx * nz + p] = P.p[q][x + nx * y + (size_t)nxy * z];
+            gy[(size_t)q * nx * nz + p] = P.p[q][x + ys * y + zs * z];
         }
     } else {
         if (p >= (long long)nxy) return;
@@ -477,7 +488,7 @@ __global__ __launch_bounds__(256) void k_pack_faces(Pops<S> P, int nx, int ny, i
         for (int q = 0; q < 18; q++) {
             if (cz(q) == 0) continue;
             const int z = cz(q) > 0 ? nz - 1 : 0;
-            gz[(size_t)q * nxy + p] = P.p[q][p + (size_t)nxy * z];
+            gz[(size_t)q * nxy + p] = P.p[q][p % nx + ys * (p / nx) + zs * z];
         }
     }
 }
@@ -488,7 +499,7 @@ static hipError_t pack_faces_t(const Lattice &L, int cur, void *g, hipStream_t s
     if ((long long)L.nx * L.nz > mx) mx = (long long)L.nx * L.nz;
     if ((long long)L.nx * L.ny > mx) mx = (long long)L.nx * L.ny;
     hipLaunchKernelGGL((k_pack_faces<S>), dim3((unsigned)((mx + 255) / 256), 3), dim3(256), 0, s,
-                       pops_of<S>(L, cur), L.nx, L.ny, L.nz, (S *)g);
+                       pops_of<S>(L, cur), L.nx, L.ny, L.nz, L.ystride, L.zstride, (S *)g);
     return hipGetLastError();
 }
 
@@ -496,31 +507,30 @@ static hipError_t pack_faces_t(const Lattice &L, int cur, void *g, hipStream_t s
 // Host transfers go through a staging buffer in chunks of `count` sites
 // starting at interior site i0.
 template <typename S, typename C>
-__global__ __launch_bounds__(256) void k_plane_from_f64(S *dst, const double *src,
-                                                        long long count) {
+__global__ __launch_bounds__(256) void k_plane_from_f64(S *dst, SiteMap m, long long i0,
+                                                        const double *src, long long count) {
     const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
-    if (i < count) dst[i] = Storage<S>::template store<C>((C)src[i]);
+    if (i < count) dst[site_elem(m, i0 + i)] = Storage<S>::template store<C>((C)src[i]);
 }
 template <typename S, typename C>
-__global__ __launch_bounds__(256) void k_plane_to_f64(const S *src, double *dst, long long count) {
+__global__ __launch_bounds__(256) void k_plane_to_f64(const S *src, SiteMap m, long long i0,
+                                                      double *dst, long long count) {
     const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
-    if (i < count) dst[i] = (double)Storage<S>::template load<C>(src[i]);
+    if (i < count) dst[i] = (double)Storage<S>::template load<C>(src[site_elem(m, i0 + i)]);
 }
 
 template <typename S, typename C>
 static hipError_t from_f64_t(const Lattice &L, int copy, int q, const double *src, long long i0,
                              long long count, hipStream_t s) {
-    S *dst = (S *)L.pop[copy][q] + (size_t)L.zoff * L.nx * L.ny + i0;
     hipLaunchKernelGGL((k_plane_from_f64<S, C>), dim3((unsigned)((count + 255) / 256)),
-                       dim3(256), 0, s, dst, src, count);
+                       dim3(256), 0, s, (S *)L.pop[copy][q], site_map(L), i0, src, count);
     return hipGetLastError();
 }
 template <typename S, typename C>
 static hipError_t to_f64_t(const Lattice &L, int copy, int q, double *dst, long long i0,
                            long long count, hipStream_t s) {
-    const S *src = (const S *)L.pop[copy][q] + (size_t)L.zoff * L.nx * L.ny + i0;
     hipLaunchKernelGGL((k_plane_to_f64<S, C>), dim3((unsigned)((count + 255) / 256)),
-                       dim3(256), 0, s, src, dst, count);
+                       dim3(256), 0, s, (const S *)L.pop[copy][q], site_map(L), i0, dst, count);
     return hipGetLastError();
 }
 
@@ -529,7 +539,7 @@ static hipError_t to_f64_t(const Lattice &L, int copy, int q, double *dst, long 
 template <typename S, typename C>
 __global__ __launch_bounds__(256) void k_init_equilibrium(Pops<S> P, const double *u,
                                                           long long ld, long long count,
-                                                          long long dst0, C cte1) {
+                                                          SiteMap m, long long i0, C cte1) {
     const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
     if (i >= count) return;
     const C rf = 3, qf = 1.5, tre = 3, crho = 1, zero = 0;
@@ -559,7 +569,7 @@ __global__ __launch_bounds__(256) void k_init_equilibrium(Pops<S> P, const doubl
 #pragma unroll
     for (int q = 0; q < 19; q++) {
         const C w = (q == 18) ? p0 : ((cx(q) * cx(q) + cy(q) * cy(q) + cz(q) * cz(q)) == 1 ? p1 : p2);
-        P.p[q][dst0 + i] = Storage<S>::template store<C>(crho * w * (cte1 + c[q]));
+        P.p[q][site_elem(m, i0 + i)] = Storage<S>::template store<C>(crho * w * (cte1 + c[q]));
     }
 }
 
@@ -568,8 +578,8 @@ static hipError_t init_eq_t(const Lattice &L, int copy, const double *u, long lo
                             long long i0, long long count, hipStream_t s) {
     // init.F90 uses cte1 = 0 when shifted (SRC/init.F90:65-69)
     hipLaunchKernelGGL((k_init_equilibrium<S, C>), dim3((unsigned)((count + 255) / 256)),
-                       dim3(256), 0, s, pops_of<S>(L, copy), u, ld, count,
-                       (long long)L.zoff * L.nx * L.ny + i0, (C)(L.shifted ? 0.0 : 1.0));
+                       dim3(256), 0, s, pops_of<S>(L, copy), u, ld, count, site_map(L), i0,
+                       (C)(L.shifted ? 0.0 : 1.0));
     return hipGetLastError();
 }
 
@@ -577,15 +587,15 @@ static hipError_t init_eq_t(const Lattice &L, int copy, const double *u, long lo
 // populations of the box [x0,x1) x [y0,y1) x [z0,z1) as doubles,
 // out[q][z][y][x] (x fastest), for host-side probes and slices
 template <typename S, typename C>
-__global__ __launch_bounds__(256) void k_gather_box(Pops<S> P, Box b, int nx, int ny, int zoff,
-                                                    double *out) {
+__global__ __launch_bounds__(256) void k_gather_box(Pops<S> P, Box b, long long ys, long long zs,
+                                                    int zoff, double *out) {
     const long long bx = b.x1 - b.x0, by = b.y1 - b.y0, bz = b.z1 - b.z0;
     const long long n = bx * by * bz;
     const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
     if (i >= n) return;
     const int x = b.x0 + (int)(i % bx), y = b.y0 + (int)((i / bx) % by),
               z = b.z0 + (int)(i / (bx * by));
-    const size_t site = (size_t)x + (size_t)nx * (y + (size_t)ny * (z + zoff));
+    const size_t site = (size_t)x + (size_t)ys * y + (size_t)zs * (z + zoff);
 #pragma unroll
     for (int q = 0; q < 19; q++)
         out[(size_t)q * n + i] = (double)Storage<S>::template load<C>(P.p[q][site]);
@@ -597,7 +607,7 @@ static hipError_t gather_box_t(const Lattice &L, int cur, const Box &b, double *
     const long long n = b.sites();
     if (n <= 0) return hipSuccess;
     hipLaunchKernelGGL((k_gather_box<S, C>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s,
-                       pops_of<S>(L, cur), b, L.nx, L.ny, L.zoff, out);
+                       pops_of<S>(L, cur), b, L.ystride, L.zstride, L.zoff, out);
     return hipGetLastError();
 }
 
