@@ -146,6 +146,37 @@ __device__ __forceinline__ gptr_t<S> plane_ptr(S *p, size_t e) {
     return r;
 }
 
+// Raw buffer access (buffer_load/store with a 128-bit resource in scalar
+// registers, a 32-bit vector offset and a 32-bit scalar offset): used where
+// the 19 populations lie at equal byte distances from population 0 (the
+// interleaved layout), so one resource per z plane serves all of them and
+// the per-population distance is a loop-invariant scalar offset.  Stride 0,
+// 2^32 - 1 records, 32-bit data format (word 3 of the CDNA descriptor).
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t raw_rsrc(const void *base) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), (short)0, -1, 0x00020000);
+}
+template <typename S>
+__device__ __forceinline__ S buf_ld(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
+    if constexpr (sizeof(S) == 2)
+        return __builtin_bit_cast(S, (uint16_t)__builtin_amdgcn_raw_buffer_load_b16(r, voff, soff, 0));
+    else if constexpr (sizeof(S) == 4)
+        return __builtin_bit_cast(S, (uint32_t)__builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
+    else
+        return __builtin_bit_cast(S, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+}
+template <typename S>
+__device__ __forceinline__ void buf_st(S v, __amdgpu_buffer_rsrc_t r, uint32_t voff,
+                                       uint32_t soff) {
+    if constexpr (sizeof(S) == 2)
+        __builtin_amdgcn_raw_buffer_store_b16(__builtin_bit_cast(uint16_t, v), r, voff, soff, 0);
+    else if constexpr (sizeof(S) == 4)
+        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), r, voff, soff, 0);
+    else
+        __builtin_amdgcn_raw_buffer_store_b64(
+            __builtin_bit_cast(unsigned int __attribute__((ext_vector_type(2))), v), r, voff,
+            soff, 0);
+}
+
 // whether the in-plane byte offsets (x + y ystride, plus the population
 // offsets of the slab plane buffer's pointers) fit 32 bits
 inline bool offsets_fit_32(const Lattice &L, size_t elem_bytes) {
@@ -311,6 +342,7 @@ struct Geo2 {
     int tiles_x, tiles_y, zc;
     int zlo, zhi, zoff, periodic_z;
     long long ystride, zstride;  // element pitches of rows and planes (as Geo)
+    unsigned pstride;            // BUF: bytes from population q to q+1
     void *faces;  // FACES: face images of the first step (k_pack_faces layout)
 };
 
@@ -329,6 +361,7 @@ static Geo2 make_geo2(const Lattice &L, int tx, int ty, int zc, int *nblocks) {
     g.zhi = g.periodic_z ? L.nz : L.nz - 2;
     g.ystride = L.ystride;
     g.zstride = L.zstride;
+    g.pstride = (unsigned)((const char *)L.pop[0][1] - (const char *)L.pop[0][0]);
     g.faces = nullptr;
     *nblocks = g.tiles_x * g.tiles_y * ((g.zhi - g.zlo + g.zc - 1) / g.zc);
     return g;
@@ -392,10 +425,14 @@ struct CollideOp {
 // FACES (periodic lattice): also write the face planes of the first step's
 // state to g.faces, in the layout of k_pack_faces, for the dissipation that
 // reads the stale halo images of the reference (BGK_FLAG_REFCOMPAT_HALOS).
-template <typename S, typename C, bool O32, int TX, int TY, class Op, bool FACES = false>
+// BUF (with O32): populations at equal distances g.pstride (the lattice's
+// interleaved copies); accesses go through one buffer resource per plane.
+template <typename S, typename C, bool O32, int TX, int TY, class Op, bool FACES = false,
+          bool BUF = false>
 __device__ __forceinline__ void two_step_pass(const Pops<S> &src, const Pops<S> &dst,
                                               const Geo2 &g, const Op &op,
                                               S (*lds)[TY + 2][TX + 2]) {
+    static_assert(!BUF || O32, "buffer access takes 32-bit offsets");
     using O = off_t_<O32>;
     using X_t = typename Op::Extra;
     constexpr int SGN = Op::SGN;
@@ -460,6 +497,17 @@ __device__ __forceinline__ void two_step_pass(const Pops<S> &src, const Pops<S> 
         const int zp = !g.periodic_z ? p + 1 : ((p == g.nz - 1) ? 0 : p + 1);
         const size_t Z[3] = {(size_t)(zm + g.zoff) * plane, (size_t)(p + g.zoff) * plane,
                              (size_t)(zp + g.zoff) * plane};
+        if constexpr (BUF) {
+            const S *b0 = src.p[0];
+            const __amdgpu_buffer_rsrc_t R[3] = {raw_rsrc(b0 + Z[0]), raw_rsrc(b0 + Z[1]),
+                                                 raw_rsrc(b0 + Z[2])};
+#pragma unroll
+            for (int q = 0; q < 19; q++)
+                raw[q] = buf_ld<S>(R[1 - SGN * cz(q)], X[1 - SGN * cx(q)] + Y[1 - SGN * cy(q)],
+                                   (uint32_t)q * g.pstride);
+            xr = op.extra_first(site_of(p));
+            return;
+        }
 #pragma unroll
         for (int q = 0; q < 19; q++) {
             // opaque per plane, so that the compiler does not keep the 64-bit
@@ -545,12 +593,20 @@ __device__ __forceinline__ void two_step_pass(const Pops<S> &src, const Pops<S> 
             }
             C b[19];
             op.second(f2, b, x2, site_of(p2));
-            O c = X[1] + Y[1];
-            asm volatile("" : "+v"(c));
             const size_t zo = (size_t)(p2 + g.zoff) * plane;
+            if constexpr (BUF) {
+                const __amdgpu_buffer_rsrc_t R = raw_rsrc(dst.p[0] + zo);
 #pragma unroll
-            for (int q = 0; q < 19; q++)
-                st_at(plane_ptr(dst.p[q], zo), c, Storage<S>::template store<C>(b[q]));
+                for (int q = 0; q < 19; q++)
+                    buf_st<S>(Storage<S>::template store<C>(b[q]), R, X[1] + Y[1],
+                              (uint32_t)q * g.pstride);
+            } else {
+                O c = X[1] + Y[1];
+                asm volatile("" : "+v"(c));
+#pragma unroll
+                for (int q = 0; q < 19; q++)
+                    st_at(plane_ptr(dst.p[q], zo), c, Storage<S>::template store<C>(b[q]));
+            }
         }
         lds_barrier();
         if (active) {
@@ -591,7 +647,7 @@ template <typename S, typename C, bool O32, int TX, int TY, int MINB, bool TAPE,
 __global__ __launch_bounds__(collide2_threads(TX, TY), MINB) void k_collide2(
     Pops<S> src, Pops<S> dst, Geo2 g, CollideOp<C, TAPE, SH> op) {
     __shared__ S lds[20][TY + 2][TX + 2];
-    two_step_pass<S, C, O32, TX, TY, CollideOp<C, TAPE, SH>, FACES>(src, dst, g, op, lds);
+    two_step_pass<S, C, O32, TX, TY, CollideOp<C, TAPE, SH>, FACES, O32>(src, dst, g, op, lds);
 }
 
 }  // namespace bgk
