@@ -379,7 +379,8 @@ def main():
         "config": {"workload": f"{a.size}^3 periodic D3Q19 BGK HIT, omega=1.95, FP32 storage "
                                f"and collide, diagno+dissipation every {a.diag_every} steps",
                    "lattice": [a.size] * 3, "omega": 1.95, "diag_every": a.diag_every,
-                   "layout": "SoA, halo-free, periodic wrap in index math"},
+                   "layout": "per copy [z][y][q][x] (19 population rows of a y line side by "
+                             "side), halo-free, periodic wrap in index math"},
         "roofline": roofline(main_run),
         "cpu_baseline": cpu,
         "cpu_samples": cpu_more,

@@ -144,8 +144,8 @@ hipError_t timing_end(bgk_ctx *c, int cls, hipStream_t s, hipEvent_t begin);
 // fused pull-stream + collide from copy `src` to copy 1-src over `box`
 hipError_t launch_collide(const Lattice &L, int src, const Box &box, double omega,
                           hipStream_t s);
-// two time steps in one pass from copy `src` into copy 1-src (a rest array per
-// copy); only when two_step_ok(L).  faces (periodic layout, may be null):
+// two time steps in one pass from copy `src` into copy 1-src; only when
+// two_step_ok(L).  faces (periodic layout, may be null):
 // receives the face images of the intermediate step (k_pack_faces layout).
 bool two_step_ok(const Lattice &L);
 hipError_t launch_collide2(const Lattice &L, int src, double omega, void *faces,

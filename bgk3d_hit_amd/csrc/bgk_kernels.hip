@@ -65,8 +65,8 @@ static hipError_t collide2_o(const Lattice &L, int src, double omega, void *face
     return L.shifted ? collide2_sh<S, C, O32, true>(L, src, omega, faces, s)
                      : collide2_sh<S, C, O32, false>(L, src, omega, faces, s);
 }
-// two steps from copy `src` into copy 1-src (periodic layout, separate rest
-// arrays per copy)
+// two steps from copy `src` into copy 1-src (periodic lattice, or the bulk
+// planes of a slab)
 template <typename S, typename C>
 static hipError_t collide2_t(const Lattice &L, int src, double omega, void *faces,
                              hipStream_t s) {
