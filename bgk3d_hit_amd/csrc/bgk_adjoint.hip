@@ -58,7 +58,7 @@ __global__ __launch_bounds__(256) void k_lambda_final(Pops<S> P, SiteMap m, C *M
         tx = tx < 0 ? tx + nx : (tx >= nx ? tx - nx : tx);
         ty = ty < 0 ? ty + ny : (ty >= ny ? ty - ny : ty);
         tz = tz < 0 ? tz + nz : (tz >= nz ? tz - nz : tz);
-        M[(size_t)q * nsite + tx + (size_t)nx * (ty + (size_t)ny * tz)] =
+        M[(size_t)q * nx + tx + (size_t)m.ystride * ty + (size_t)m.zstride * tz] =
             (ux * (C)cx(q) + uy * (C)cy(q) + uz * (C)cz(q) - usq) / rho;
     }
 }
@@ -93,7 +93,8 @@ template <typename C>
 __global__ __launch_bounds__(256) void k_adjoint_step(const C *__restrict__ Min,
                                                       C *__restrict__ Mout,
                                                       const C *__restrict__ tape, int nx, int ny,
-                                                      int nz, int tiles_x, int tiles_y, C omega) {
+                                                      int nz, long long ys, long long zs,
+                                                      int tiles_x, int tiles_y, C omega) {
     const int t = xcd_remap((int)blockIdx.x, (int)gridDim.x);
     const int tx = t % tiles_x;
     const int r = t / tiles_x;
@@ -107,18 +108,19 @@ __global__ __launch_bounds__(256) void k_adjoint_step(const C *__restrict__ Min,
     const int xm = (x == 0) ? nx - 1 : x - 1, xp = (x == nx - 1) ? 0 : x + 1;
     const int ym = (y == 0) ? ny - 1 : y - 1, yp = (y == ny - 1) ? 0 : y + 1;
     const int zm = (z == 0) ? nz - 1 : z - 1, zp = (z == nz - 1) ? 0 : z + 1;
-    const int X[3] = {xm, x, xp};
-    const int Y[3] = {ym * nx, y * nx, yp * nx};
-    const int Z[3] = {zm * nxy, z * nxy, zp * nxy};
+    const long long X[3] = {xm, x, xp};
+    const long long Y[3] = {ym * ys, y * ys, yp * ys};
+    const long long Z[3] = {zm * zs, z * zs, zp * zs};
     C lam[19];
 #pragma unroll
     for (int q = 0; q < 19; q++)
-        lam[q] = Min[(size_t)q * N + X[1 + cx(q)] + Y[1 + cy(q)] + Z[1 + cz(q)]];
-    const int c = x + Y[1] + Z[1];
+        lam[q] = Min[(size_t)q * nx + X[1 + cx(q)] + Y[1 + cy(q)] + Z[1 + cz(q)]];
+    const size_t c = (size_t)(x + Y[1] + Z[1]);                   // lambda element
+    const size_t i = (size_t)x + (size_t)nx * y + (size_t)nxy * z;  // tape site
     C mu[19];
-    adjoint_site<C>(lam, tape[c], tape[N + c], tape[2 * N + c], omega, mu);
+    adjoint_site<C>(lam, tape[i], tape[N + i], tape[2 * N + i], omega, mu);
 #pragma unroll
-    for (int q = 0; q < 19; q++) Mout[(size_t)q * N + c] = mu[q];
+    for (int q = 0; q < 19; q++) Mout[(size_t)q * nx + c] = mu[q];
 }
 
 // Two adjoint steps in one pass (the two-step pass of bgk_collide.h with the
@@ -155,14 +157,15 @@ __global__ __launch_bounds__(collide2_threads(TX, TY), MINB) void k_adjoint2(Pop
                                                                              Pops<C> dst, Geo2 g,
                                                                              AdjointOp<C> op) {
     __shared__ C lds[20][TY + 2][TX + 2];
-    two_step_pass<C, C, O32, TX, TY>(src, dst, g, op, lds);
+    two_step_pass<C, C, O32, TX, TY, AdjointOp<C>, false, O32>(src, dst, g, op, lds);
 }
 
 // dJ/du0 = sum_i lambda^0_i w_i (3 c_i + 9 (c_i.u0) c_i - 3 u0),
 // lambda^0_i(x) = M^0_i(x + c_i)
 template <typename C>
 __global__ __launch_bounds__(256) void k_grad_u0(const C *M, const double *u0, double *grad,
-                                                 int nx, int ny, int nz) {
+                                                 int nx, int ny, int nz, long long ys,
+                                                 long long zs) {
     const long long N = (long long)nx * ny * nz;
     const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
     if (i >= N) return;
@@ -176,7 +179,7 @@ __global__ __launch_bounds__(256) void k_grad_u0(const C *M, const double *u0, d
         sy = sy < 0 ? sy + ny : (sy >= ny ? sy - ny : sy);
         sz = sz < 0 ? sz + nz : (sz >= nz ? sz - nz : sz);
         const double L =
-            (double)M[(size_t)q * N + sx + (size_t)nx * (sy + (size_t)ny * sz)] * wq(q);
+            (double)M[(size_t)q * nx + sx + (size_t)ys * sy + (size_t)zs * sz] * wq(q);
         const double cu = cx(q) * ux + cy(q) * uy + cz(q) * uz;
         gx += L * (3.0 * cx(q) + 9.0 * cu * cx(q) - 3.0 * ux);
         gy += L * (3.0 * cy(q) + 9.0 * cu * cy(q) - 3.0 * uy);
@@ -280,12 +283,12 @@ hipError_t launch_adjoint2_o(bgk_ctx *c, const C *lin, C *lout, const C *tape0,
     using T = Tile2<C, C, O32>;
     int nblocks;
     Geo2 g = make_geo2(L, T::X, T::Y, two_step_zc(L, T::X, T::Y, T::ZC), &nblocks);
-    g.ystride = L.nx;  // lambda: 19 separate arrays of nsite values
-    g.zstride = (long long)L.nx * L.ny;
+    // lambda: the lattice's interleaved layout in the compute kind
+    g.pstride = (unsigned)(L.nx * sizeof(C));
     Pops<C> a, b;
     for (int q = 0; q < 19; q++) {
-        a.p[q] = (C *)lin + (size_t)q * L.nsite;
-        b.p[q] = lout + (size_t)q * L.nsite;
+        a.p[q] = (C *)lin + (size_t)q * L.nx;
+        b.p[q] = lout + (size_t)q * L.nx;
     }
     const AdjointOp<C> op{(C)c->omega, tape0, tape1, (size_t)L.nsite};
     hipLaunchKernelGGL((k_adjoint2<C, O32, T::X, T::Y, T::MINB>), dim3(nblocks),
@@ -325,7 +328,7 @@ hipError_t launch_adjoint_step(bgk_ctx *c, const C *lin, C *lout, const C *tape)
     const Lattice &L = c->lat;
     const int tx = (L.nx + 63) / 64, ty = (L.ny + 3) / 4;
     hipLaunchKernelGGL(k_adjoint_step<C>, dim3(tx * ty * L.nz), dim3(64, 4), 0, c->s0, lin, lout,
-                       tape, L.nx, L.ny, L.nz, tx, ty, (C)c->omega);
+                       tape, L.nx, L.ny, L.nz, L.ystride, L.zstride, tx, ty, (C)c->omega);
     return hipGetLastError();
 }
 
@@ -423,7 +426,7 @@ int adjoint_t(bgk_ctx *c, const double *u0, int T, int K, int levels, double *J,
         }
     }
     hipLaunchKernelGGL(k_grad_u0<C>, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, c->s0, lin,
-                       u0, grad, L.nx, L.ny, L.nz);
+                       u0, grad, L.nx, L.ny, L.nz, L.ystride, L.zstride);
     BGK_HIP(hipGetLastError());
     BGK_HIP(hipStreamSynchronize(c->s0));
     const double bwd_ms = ms_since(t0);
