@@ -140,3 +140,13 @@ def test_hit_anisotropic_box():
     kz = np.fft.fftfreq(8, 1 / 8)[:, None, None] * 4  # kappa_z = kz * lx / lz
     div = kx * U[0] + ky * U[1] + kz * U[2]
     assert np.abs(div).max() < 1e-14 * np.abs(U).max()
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py measures one GPU; --gpus N > 1 exits with a message."""
+    import subprocess
+    import sys
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode != 0
+    assert "one GPU" in r.stdout + r.stderr
