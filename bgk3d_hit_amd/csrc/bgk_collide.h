@@ -140,7 +140,7 @@ __device__ __forceinline__ void st_at(__attribute__((address_space(1))) S *base,
 template <typename S>
 using gptr_t = __attribute__((address_space(1))) S *;
 template <typename S>
-__device__ __forceinline__ gptr_t<S> plane_ptr(S *p, size_t e) {
+__device__ __forceinline__ gptr_t<S> plane_ptr(S *p, ptrdiff_t e) {
     gptr_t<S> r = (gptr_t<S>)(p + e);
     asm volatile("" : "+s"(r));
     return r;
@@ -220,7 +220,8 @@ __global__ __launch_bounds__(Tile<S>::X * Tile<S>::Y) void k_collide(Pops<S> src
     const O X[3] = {(O)xm * es, (O)x * es, (O)xp * es};
     const O Y[3] = {(O)ym * (O)g.ystride * es, (O)y * (O)g.ystride * es,
                     (O)yp * (O)g.ystride * es};
-    const size_t Z[3] = {(size_t)zm * g.zstride, (size_t)zs * g.zstride, (size_t)zp * g.zstride};
+    const ptrdiff_t Z[3] = {(ptrdiff_t)zm * g.zstride, (ptrdiff_t)zs * g.zstride,
+                            (ptrdiff_t)zp * g.zstride};
 
     C f[19];
 #pragma unroll
@@ -480,7 +481,7 @@ __device__ __forceinline__ void two_step_pass(const Pops<S> &src, const Pops<S> 
     const O X[3] = {(O)xm * es, (O)gx * es, (O)xp * es};
     const O Y[3] = {(O)ym * (O)g.ystride * es, (O)gy * (O)g.ystride * es,
                     (O)yp * (O)g.ystride * es};
-    const size_t plane = (size_t)g.zstride;
+    const ptrdiff_t plane = (ptrdiff_t)g.zstride;
 
     // z in [zlo-1, zhi]; a slab's planes need no wrap
     auto wrapz = [&](int z) {
@@ -495,8 +496,8 @@ __device__ __forceinline__ void two_step_pass(const Pops<S> &src, const Pops<S> 
         const int p = wrapz(z);
         const int zm = !g.periodic_z ? p - 1 : ((p == 0) ? g.nz - 1 : p - 1);
         const int zp = !g.periodic_z ? p + 1 : ((p == g.nz - 1) ? 0 : p + 1);
-        const size_t Z[3] = {(size_t)(zm + g.zoff) * plane, (size_t)(p + g.zoff) * plane,
-                             (size_t)(zp + g.zoff) * plane};
+        const ptrdiff_t Z[3] = {(ptrdiff_t)(zm + g.zoff) * plane, (ptrdiff_t)(p + g.zoff) * plane,
+                                (ptrdiff_t)(zp + g.zoff) * plane};
         if constexpr (BUF) {
             const S *b0 = src.p[0];
             const __amdgpu_buffer_rsrc_t R[3] = {raw_rsrc(b0 + Z[0]), raw_rsrc(b0 + Z[1]),
@@ -593,7 +594,7 @@ __device__ __forceinline__ void two_step_pass(const Pops<S> &src, const Pops<S> 
             }
             C b[19];
             op.second(f2, b, x2, site_of(p2));
-            const size_t zo = (size_t)(p2 + g.zoff) * plane;
+            const ptrdiff_t zo = (ptrdiff_t)(p2 + g.zoff) * plane;
             if constexpr (BUF) {
                 const __amdgpu_buffer_rsrc_t R = raw_rsrc(dst.p[0] + zo);
 #pragma unroll

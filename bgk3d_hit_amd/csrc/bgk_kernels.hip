@@ -429,10 +429,10 @@ static hipError_t unpack_z_t(const Lattice &L, int cur, const void *recv, hipStr
     return unpack_planes_t<S>(L, L.pop[cur], 0, L.nz + 1, recv, s);
 }
 
-// one step over the slab box `box` from population arrays `src` into `dst`,
-// where interior plane z is stored at plane z + src_off (z + dst_off): the
-// slab's two-step boundary path, which takes single steps through a small
-// plane buffer
+// one step over the slab box `box` from populations `src` into `dst` (both
+// in the lattice's layout), where interior plane z is stored at plane
+// z + src_off (z + dst_off): the slab's two-step boundary path, which takes
+// single steps through a small plane buffer
 template <typename S, typename C>
 static hipError_t collide_planes_t(const Lattice &L, void *const *src, int src_off,
                                    void *const *dst, int dst_off, const Box &box, double omega,
@@ -447,9 +447,14 @@ static hipError_t collide_planes_t(const Lattice &L, void *const *src, int src_o
         a.p[q] = (S *)src[q] + (ptrdiff_t)src_off * (ptrdiff_t)L.zstride;
         b.p[q] = (S *)dst[q] + (ptrdiff_t)dst_off * (ptrdiff_t)L.zstride;
     }
-    // 64-bit offsets: the shifted base pointers may lie below an array
-    hipLaunchKernelGGL((k_collide<S, C, false>), dim3(ntiles), dim3(Tile<S>::X, Tile<S>::Y), 0, s,
-                       a, b, g, (C)omega, (C)(L.shifted ? 1.0 : 0.0));
+    // the shifted pointers may lie below an allocation; the kernel adds the
+    // (signed) plane offset to them and only the in-plane part is an offset
+    if (offsets_fit_32(L, sizeof(S)))
+        hipLaunchKernelGGL((k_collide<S, C, true>), dim3(ntiles), dim3(Tile<S>::X, Tile<S>::Y), 0,
+                           s, a, b, g, (C)omega, (C)(L.shifted ? 1.0 : 0.0));
+    else
+        hipLaunchKernelGGL((k_collide<S, C, false>), dim3(ntiles), dim3(Tile<S>::X, Tile<S>::Y), 0,
+                           s, a, b, g, (C)omega, (C)(L.shifted ? 1.0 : 0.0));
     return hipGetLastError();
 }
 
