@@ -23,7 +23,9 @@ factor by which it beats the bandwidth bound of any one-step-per-launch
 kernel.  `traffic` is the HBM bytes per launch measured by rocprofv3
 FETCH_SIZE / WRITE_SIZE runs of this script (profiles/collide_traffic.json,
 null when absent or for another lattice size); `hbm_actual_gbs` = traffic /
-launch time is the kernel's own DRAM rate.
+launch time is the kernel's own DRAM rate, and `copy_ceiling_gbs` the rate
+of a plain copy of the same 38-stream pattern in the same layout
+(bin/stream_ceiling, profiles/stream_ceiling_512cubed.jsonl).
 
 `cpu_baseline` times the CPU oracle (plain C/OpenMP restatement of the
 reference Fortran) on a bounded sample on the host cores.
@@ -281,6 +283,23 @@ def load_traffic(kind, two_step, nsite):
         return None
 
 
+def copy_ceiling(nsite):
+    """GB/s of a plain 19-in/19-out copy in the interleaved layout at 512^3
+    (bin/stream_ceiling), the achievable rate for the 38-stream pattern."""
+    if nsite != 512 ** 3:
+        return None
+    p = os.path.join(ROOT, "profiles", "stream_ceiling_512cubed.jsonl")
+    try:
+        with open(p) as f:
+            for line in f:
+                d = json.loads(line)
+                if d.get("kernel") == "copy19_interleaved":
+                    return d.get("gbs")
+    except (OSError, ValueError):
+        pass
+    return None
+
+
 def roofline(r):
     traffic = load_traffic(r["kind"], r["two_step"], r["nsite"])
     out = {"bound": "hbm", "achieved": round(r["achieved_gbs"], 1),
@@ -294,6 +313,10 @@ def roofline(r):
         act = traffic / (r["collide_avg_ms"] * 1e-3) / 1e9
         out["hbm_actual_gbs"] = round(act, 1)
         out["hbm_actual_frac"] = round(act / HBM_PEAK_GBS, 4)
+    ceil = copy_ceiling(r["nsite"])
+    out["copy_ceiling_gbs"] = ceil
+    out["hbm_actual_frac_of_copy"] = (round(out["hbm_actual_gbs"] / ceil, 4)
+                                      if ceil and out["hbm_actual_gbs"] else None)
     return out
 
 
