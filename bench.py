@@ -73,7 +73,8 @@ def run_gpu(kind, n, steps, warmup, diag_every, omega=1.95, shape=None, layout="
             lat.step(k)
             done += k
             if done % diag_every == 0:
-                diag.append((done + warmup, *lat.diagno(), *lat.dissipation(nu)))
+                d5, s2 = lat.diagnostics(nu)  # diagno + dissipation, one pass
+                diag.append((done + warmup, *d5, *s2))
         lat.synchronize()
         wall = time.perf_counter() - t0
         t = lat.timing(reset=True)

@@ -48,6 +48,7 @@ SIGNATURES = {
     "bgk_step": [_p, _c_int],
     "bgk_diagno": [_p, _c_int, _dp],
     "bgk_dissipation": [_p, _c_double, _c_int, _dp],
+    "bgk_diagnostics": [_p, _c_double, _c_int, _dp],
     "bgk_get_box": [_p, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _dp],
     "bgk_checksum": [_p, ctypes.POINTER(ctypes.c_uint64)],
     "bgk_synchronize": [_p],

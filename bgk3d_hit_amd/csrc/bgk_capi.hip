@@ -448,11 +448,12 @@ int bgk_create(int lx, int ly, int lz, int storage, int layout, int flags, doubl
     long long nb = (L.nsite + 255) / 256;
     c->red_blocks = (int)std::min<long long>(std::max<long long>(nb, 1), 2048);  // checksum
     const int nparts = std::max(c->red_blocks, reduction_parts(L));
-    if ((e = hipMalloc(&c->red_partials, sizeof(double) * 8 * nparts)) != hipSuccess)
+    // two sets of 8 partial sums per block (diagno and dissipation in one pass)
+    if ((e = hipMalloc(&c->red_partials, sizeof(double) * 16 * nparts)) != hipSuccess)
         return bail(hip_fail(e, "hipMalloc(reduction)"));
-    if ((e = hipMalloc(&c->red_out, sizeof(double) * 8)) != hipSuccess)
+    if ((e = hipMalloc(&c->red_out, sizeof(double) * 16)) != hipSuccess)
         return bail(hip_fail(e, "hipMalloc(reduction)"));
-    if ((e = hipHostMalloc(&c->red_host, sizeof(double) * 8, hipHostMallocDefault)) !=
+    if ((e = hipHostMalloc(&c->red_host, sizeof(double) * 16, hipHostMallocDefault)) !=
         hipSuccess)
         return bail(hip_fail(e, "hipHostMalloc(reduction)"));
     const long long chunk = std::min(L.nsite, CHUNK);
@@ -651,6 +652,47 @@ int bgk_step(bgk_ctx *c, int nsteps) {
     return BGK_OK;
 }
 
+// diagno's outputs from the 6 reduced sums v (SRC/diagno.F90:110-121)
+static int diagno_finish(const Lattice &L, int mode, const double *v, double out[5]) {
+    // SRC/diagno.F90:110 leaves ztot = ytot + zj(l,m,n) after the loop
+    const double ztot = (mode == BGK_DIAG_REFCOMPAT) ? v[2] + v[5] : v[3];
+    const double tot[5] = {v[0], v[1], v[2], ztot, v[4]};
+    for (int k = 0; k < 5; k++)  // SRC/diagno.F90:117-121 division order
+        out[k] = ((tot[k] / (double)L.nx) / (double)L.ny) / (double)L.nz;
+    for (int k = 0; k < 5; k++)
+        if (!std::isfinite(out[k]))
+            return fail(BGK_ENONFINITE, "diagno: non-finite moments (populations diverged)");
+    return BGK_OK;
+}
+
+// where the dissipation's pulled gather finds the sources outside the box:
+// *gc = face images (refcompat, periodic) or null (index wrap / slab ghosts,
+// refreshed first in consistent mode)
+static int dissipation_sources(bgk_ctx *c, int mode, const void **gc) {
+    const Lattice &L = c->lat;
+    *gc = nullptr;
+    if (L.layout == BGK_LAYOUT_SLAB_Z) {
+        // slab ghosts as the last exchange left them are the reference's
+        // view; consistent mode refreshes them first
+        if (mode == BGK_DIAG_CONSISTENT) return halo_exchange(c, c->s0);
+    } else if (mode == BGK_DIAG_REFCOMPAT) {
+        if (!(c->flags & BGK_FLAG_REFCOMPAT_HALOS))
+            return fail(BGK_ESTATE,
+                        "refcompat dissipation needs a context created with "
+                        "BGK_FLAG_REFCOMPAT_HALOS");
+        *gc = c->gcache[c->cur];
+    }
+    return BGK_OK;
+}
+
+static int dissipation_finish(const double *v, double out[2]) {
+    out[0] = v[0];
+    out[1] = v[1];
+    if (!std::isfinite(out[0]) || !std::isfinite(out[1]))
+        return fail(BGK_ENONFINITE, "dissipation: non-finite sums (populations diverged)");
+    return BGK_OK;
+}
+
 int bgk_diagno(bgk_ctx *c, int mode, double out[5]) {
     int r = check_ctx(c);
     if (r) return r;
@@ -665,16 +707,7 @@ int bgk_diagno(bgk_ctx *c, int mode, double out[5]) {
     BGK_HIP(hipMemcpyAsync(c->red_host, c->red_out, 8 * sizeof(double), hipMemcpyDeviceToHost,
                            c->s0));
     BGK_HIP(hipStreamSynchronize(c->s0));
-    const double *v = c->red_host;
-    // SRC/diagno.F90:110 leaves ztot = ytot + zj(l,m,n) after the loop
-    const double ztot = (mode == BGK_DIAG_REFCOMPAT) ? v[2] + v[5] : v[3];
-    const double tot[5] = {v[0], v[1], v[2], ztot, v[4]};
-    for (int k = 0; k < 5; k++)  // SRC/diagno.F90:117-121 division order
-        out[k] = ((tot[k] / (double)L.nx) / (double)L.ny) / (double)L.nz;
-    for (int k = 0; k < 5; k++)
-        if (!std::isfinite(out[k]))
-            return fail(BGK_ENONFINITE, "diagno: non-finite moments (populations diverged)");
-    return BGK_OK;
+    return diagno_finish(L, mode, c->red_host, out);
 }
 
 int bgk_dissipation(bgk_ctx *c, double svisc, int mode, double out[2]) {
@@ -685,20 +718,7 @@ int bgk_dissipation(bgk_ctx *c, double svisc, int mode, double out[2]) {
         return fail(BGK_EINVAL, "unknown diagnostic mode");
     const Lattice &L = c->lat;
     const void *gc = nullptr;
-    if (L.layout == BGK_LAYOUT_SLAB_Z) {
-        // slab ghosts as the last exchange left them are the reference's
-        // view; consistent mode refreshes them first
-        if (mode == BGK_DIAG_CONSISTENT) {
-            r = halo_exchange(c, c->s0);
-            if (r) return r;
-        }
-    } else if (mode == BGK_DIAG_REFCOMPAT) {
-        if (!(c->flags & BGK_FLAG_REFCOMPAT_HALOS))
-            return fail(BGK_ESTATE,
-                        "refcompat dissipation needs a context created with "
-                        "BGK_FLAG_REFCOMPAT_HALOS");
-        gc = c->gcache[c->cur];
-    }
+    if ((r = dissipation_sources(c, mode, &gc))) return r;
     hipEvent_t ev;
     BGK_HIP(timing_begin(c, TCLS_REDUCE, c->s0, &ev));
     BGK_HIP(launch_dissipation(L, c->cur, gc, svisc, c->red_partials, c->red_out, c->s0));
@@ -706,11 +726,27 @@ int bgk_dissipation(bgk_ctx *c, double svisc, int mode, double out[2]) {
     BGK_HIP(hipMemcpyAsync(c->red_host, c->red_out, 8 * sizeof(double), hipMemcpyDeviceToHost,
                            c->s0));
     BGK_HIP(hipStreamSynchronize(c->s0));
-    out[0] = c->red_host[0];
-    out[1] = c->red_host[1];
-    if (!std::isfinite(out[0]) || !std::isfinite(out[1]))
-        return fail(BGK_ENONFINITE, "dissipation: non-finite sums (populations diverged)");
-    return BGK_OK;
+    return dissipation_finish(c->red_host, out);
+}
+
+int bgk_diagnostics(bgk_ctx *c, double svisc, int mode, double out[7]) {
+    int r = check_ctx(c);
+    if (r) return r;
+    if (!out) return fail(BGK_EINVAL, "null output");
+    if (mode != BGK_DIAG_CONSISTENT && mode != BGK_DIAG_REFCOMPAT)
+        return fail(BGK_EINVAL, "unknown diagnostic mode");
+    const Lattice &L = c->lat;
+    const void *gc = nullptr;
+    if ((r = dissipation_sources(c, mode, &gc))) return r;
+    hipEvent_t ev;
+    BGK_HIP(timing_begin(c, TCLS_REDUCE, c->s0, &ev));
+    BGK_HIP(launch_diag_diss(L, c->cur, gc, svisc, c->red_partials, c->red_out, c->s0));
+    BGK_HIP(timing_end(c, TCLS_REDUCE, c->s0, ev));
+    BGK_HIP(hipMemcpyAsync(c->red_host, c->red_out, 16 * sizeof(double), hipMemcpyDeviceToHost,
+                           c->s0));
+    BGK_HIP(hipStreamSynchronize(c->s0));
+    if ((r = diagno_finish(L, mode, c->red_host, out))) return r;
+    return dissipation_finish(c->red_host + 8, out + 5);
 }
 
 int bgk_get_box(bgk_ctx *c, int x0, int x1, int y0, int y1, int z0, int z1, double *out) {

@@ -150,6 +150,10 @@ hipError_t launch_collide(const Lattice &L, int src, const Box &box, double omeg
 bool two_step_ok(const Lattice &L);
 hipError_t launch_collide2(const Lattice &L, int src, double omega, void *faces,
                            hipStream_t s);
+// diagno and dissipation in one pass: partials holds 2 * 8 * parts doubles,
+// out 16 (out[0..7] as launch_diagno's, out[8..15] as launch_dissipation's)
+hipError_t launch_diag_diss(const Lattice &L, int cur, const void *gcache, double svisc,
+                            double *partials, double *out, hipStream_t s);
 // reductions: at-site (diagno) and pulled (dissipation)
 //   mode 0: pulled gather with periodic wrap (or slab ghosts as stored)
 //   mode 1: pulled gather reading face images from gcache (periodic layout)

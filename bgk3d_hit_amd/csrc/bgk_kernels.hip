@@ -155,51 +155,19 @@ int reduction_parts(const Lattice &L) {
     }
 }
 
-// at-site moments: v = {sum rho, sum ux, sum uy, sum uz, sum |u|^2/2,
-// uz of the last site (l,m,n)}
-template <typename S, typename C>
-__global__ __launch_bounds__(Tile<S>::X * Tile<S>::Y) void k_diagno(Pops<S> P, RedGeo g, C cte1,
-                                                                    double *partials) {
-    const int t = xcd_remap((int)blockIdx.x, (int)gridDim.x);
-    const int tx = t % g.tiles_x;
-    const int r = t / g.tiles_x;
-    const int ty = r % g.tiles_y;
-    const int zc = r / g.tiles_y;
-    const int x = tx * Tile<S>::X + (int)threadIdx.x;
-    const int y = ty * Tile<S>::Y + (int)threadIdx.y;
-    double v[RED_NV];
-#pragma unroll
-    for (int k = 0; k < RED_NV; k++) v[k] = 0.0;
-    if (x < g.nx && y < g.ny) {
-        const int z1 = min(g.nz, (zc + 1) * g.zb);
-        for (int z = zc * g.zb; z < z1; z++) {
-            const size_t site = (size_t)x + (size_t)g.ystride * y + (size_t)g.zstride * (z + g.zoff);
-            C f[19];
-#pragma unroll
-            for (int q = 0; q < 19; q++) f[q] = Storage<S>::template load<C>(P.p[q][site]);
-            C rho, ux, uy, uz;
-            moments_site<C>(f, cte1, rho, ux, uy, uz);
-            const C sk = (C)0.5 * (ux * ux + uy * uy + uz * uz);
-            v[0] += (double)rho;
-            v[1] += (double)ux;
-            v[2] += (double)uy;
-            v[3] += (double)uz;
-            v[4] += (double)sk;
-            if (x == g.nx - 1 && y == g.ny - 1 && z == g.nz - 1) v[5] = (double)uz;
-        }
-    }
-    block_sum_store<RED_NV>(v, partials + (size_t)t * RED_NV);
-}
-
-// pulled moments + Pi^neq; GC = 0: sources wrap periodically (slab: the
-// ghost planes as stored), GC = 1: sources outside the box read the recorded
-// face images gx/gy/gz (reference stale-halo semantics, periodic layout)
-template <typename S, typename C, int GC>
-__global__ __launch_bounds__(Tile<S>::X * Tile<S>::Y) void k_dissipation(Pops<S> P,
-                                                                         const S *gcache,
-                                                                         RedGeo g, C cte1,
-                                                                         C svisc,
-                                                                         double *partials) {
+// Reduction body over the block's sites (one tile, one z chunk).
+// WHAT & 1: at-site moments into vd = {sum rho, sum ux, sum uy, sum uz,
+// sum |u|^2/2, uz of the last site (l,m,n)} (SRC/diagno.F90:96-111);
+// WHAT & 2: pulled moments + Pi^neq into vs = {sum 2 svisc P, sum |u|^2/2}
+// (SRC/dissipation.F90:76-249).  GC = 0: pulled sources wrap periodically
+// (slab: the ghost planes as stored), GC = 1: sources outside the box read
+// the recorded face images gx/gy/gz (reference stale-halo semantics,
+// periodic layout).  Each sum runs over the same sites in the same order
+// whichever parts are taken, so a fused reduction equals the separate ones
+// bit for bit.
+template <typename S, typename C, int WHAT, int GC>
+__device__ __forceinline__ void reduce_body(const Pops<S> &P, const S *gcache, const RedGeo &g,
+                                            C cte1, C svisc, double *pd, double *ps) {
     const int t = xcd_remap((int)blockIdx.x, (int)gridDim.x);
     const int tx = t % g.tiles_x;
     const int r = t / g.tiles_x;
@@ -208,9 +176,9 @@ __global__ __launch_bounds__(Tile<S>::X * Tile<S>::Y) void k_dissipation(Pops<S>
     const int x = tx * Tile<S>::X + (int)threadIdx.x;
     const int y = ty * Tile<S>::Y + (int)threadIdx.y;
     const int nx = g.nx, ny = g.ny, nz = g.nz, nxy = g.nxy;
-    double v[RED_NV];
+    double vd[RED_NV], vs[RED_NV];
 #pragma unroll
-    for (int k = 0; k < RED_NV; k++) v[k] = 0.0;
+    for (int k = 0; k < RED_NV; k++) vd[k] = vs[k] = 0.0;
     if (x < nx && y < ny) {
         const int xm = (x == 0) ? nx - 1 : x - 1, xp = (x == nx - 1) ? 0 : x + 1;
         const int ym = (y == 0) ? ny - 1 : y - 1, yp = (y == ny - 1) ? 0 : y + 1;
@@ -220,49 +188,91 @@ __global__ __launch_bounds__(Tile<S>::X * Tile<S>::Y) void k_dissipation(Pops<S>
         const int z1 = min(nz, (zc + 1) * g.zb);
         for (int z = zc * g.zb; z < z1; z++) {
             const int zs = z + g.zoff;
-            int zm, zp;
-            if (g.periodic_z) {
-                zm = (z == 0) ? nz - 1 : z - 1;
-                zp = (z == nz - 1) ? 0 : z + 1;
-            } else {
-                zm = zs - 1;
-                zp = zs + 1;
-            }
-            const int Zc[3] = {zm, zs, zp};
-            const long long Z[3] = {zm * g.zstride, zs * g.zstride, zp * g.zstride};
-            C f[19];
+            if (WHAT & 1) {
+                const size_t site = (size_t)x + (size_t)Y[1] + (size_t)g.zstride * zs;
+                C f[19];
 #pragma unroll
-            for (int q = 0; q < 19; q++) {
-                S raw = P.p[q][X[1 - cx(q)] + Y[1 - cy(q)] + Z[1 - cz(q)]];
-                if (GC) {
-                    // reference ghosts: a source outside the box reads the face
-                    // image recorded when this copy was last exchanged
-                    const int sx = x - cx(q), sy = y - cy(q), sz = z - cz(q);
-                    const bool ox = (sx < 0 || sx >= nx), oy = (sy < 0 || sy >= ny);
-                    const bool oz = (sz < 0 || sz >= nz);
-                    if (ox | oy | oz) {
-                        const int wx = X[1 - cx(q)], wy = Yc[1 - cy(q)];
-                        const int wz = Zc[1 - cz(q)];
-                        const S *gx = gcache;
-                        const S *gy = gcache + (size_t)19 * ny * nz;
-                        const S *gz = gy + (size_t)19 * nx * nz;
-                        if (ox)
-                            raw = gx[(size_t)q * ny * nz + wy + (size_t)ny * wz];
-                        else if (oy)
-                            raw = gy[(size_t)q * nx * nz + wx + (size_t)nx * wz];
-                        else
-                            raw = gz[(size_t)q * nxy + wx + (size_t)nx * wy];
-                    }
-                }
-                f[q] = Storage<S>::template load<C>(raw);
+                for (int q = 0; q < 19; q++) f[q] = Storage<S>::template load<C>(P.p[q][site]);
+                C rho, ux, uy, uz;
+                moments_site<C>(f, cte1, rho, ux, uy, uz);
+                const C sk = (C)0.5 * (ux * ux + uy * uy + uz * uz);
+                vd[0] += (double)rho;
+                vd[1] += (double)ux;
+                vd[2] += (double)uy;
+                vd[3] += (double)uz;
+                vd[4] += (double)sk;
+                if (x == nx - 1 && y == ny - 1 && z == nz - 1) vd[5] = (double)uz;
             }
-            C dsite, tsite;
-            dissipation_site<C>(f, cte1, svisc, dsite, tsite);
-            v[0] += (double)dsite;
-            v[1] += (double)tsite;
+            if (WHAT & 2) {
+                int zm, zp;
+                if (g.periodic_z) {
+                    zm = (z == 0) ? nz - 1 : z - 1;
+                    zp = (z == nz - 1) ? 0 : z + 1;
+                } else {
+                    zm = zs - 1;
+                    zp = zs + 1;
+                }
+                const int Zc[3] = {zm, zs, zp};
+                const long long Z[3] = {zm * g.zstride, zs * g.zstride, zp * g.zstride};
+                C f[19];
+#pragma unroll
+                for (int q = 0; q < 19; q++) {
+                    S raw = P.p[q][X[1 - cx(q)] + Y[1 - cy(q)] + Z[1 - cz(q)]];
+                    if (GC) {
+                        // reference ghosts: a source outside the box reads the
+                        // face image recorded when this copy was last exchanged
+                        const int sx = x - cx(q), sy = y - cy(q), sz = z - cz(q);
+                        const bool ox = (sx < 0 || sx >= nx), oy = (sy < 0 || sy >= ny);
+                        const bool oz = (sz < 0 || sz >= nz);
+                        if (ox | oy | oz) {
+                            const int wx = X[1 - cx(q)], wy = Yc[1 - cy(q)];
+                            const int wz = Zc[1 - cz(q)];
+                            const S *gx = gcache;
+                            const S *gy = gcache + (size_t)19 * ny * nz;
+                            const S *gz = gy + (size_t)19 * nx * nz;
+                            if (ox)
+                                raw = gx[(size_t)q * ny * nz + wy + (size_t)ny * wz];
+                            else if (oy)
+                                raw = gy[(size_t)q * nx * nz + wx + (size_t)nx * wz];
+                            else
+                                raw = gz[(size_t)q * nxy + wx + (size_t)nx * wy];
+                        }
+                    }
+                    f[q] = Storage<S>::template load<C>(raw);
+                }
+                C dsite, tsite;
+                dissipation_site<C>(f, cte1, svisc, dsite, tsite);
+                vs[0] += (double)dsite;
+                vs[1] += (double)tsite;
+            }
         }
     }
-    block_sum_store<RED_NV>(v, partials + (size_t)t * RED_NV);
+    if (WHAT & 1) block_sum_store<RED_NV>(vd, pd + (size_t)t * RED_NV);
+    if (WHAT == 3) __syncthreads();  // block_sum_store's LDS is reused
+    if (WHAT & 2) block_sum_store<RED_NV>(vs, ps + (size_t)t * RED_NV);
+}
+
+template <typename S, typename C>
+__global__ __launch_bounds__(Tile<S>::X * Tile<S>::Y) void k_diagno(Pops<S> P, RedGeo g, C cte1,
+                                                                    double *partials) {
+    reduce_body<S, C, 1, 0>(P, nullptr, g, cte1, (C)0, partials, nullptr);
+}
+template <typename S, typename C, int GC>
+__global__ __launch_bounds__(Tile<S>::X * Tile<S>::Y) void k_dissipation(Pops<S> P,
+                                                                         const S *gcache,
+                                                                         RedGeo g, C cte1,
+                                                                         C svisc,
+                                                                         double *partials) {
+    reduce_body<S, C, 2, GC>(P, gcache, g, cte1, svisc, nullptr, partials);
+}
+// both in one pass over the populations (bgk_diagnostics)
+template <typename S, typename C, int GC>
+__global__ __launch_bounds__(Tile<S>::X * Tile<S>::Y) void k_diag_diss(Pops<S> P,
+                                                                       const S *gcache,
+                                                                       RedGeo g, C cte1,
+                                                                       C svisc, double *pd,
+                                                                       double *ps) {
+    reduce_body<S, C, 3, GC>(P, gcache, g, cte1, svisc, pd, ps);
 }
 
 template <typename S, typename C>
@@ -293,6 +303,27 @@ static hipError_t dissipation_t(const Lattice &L, int cur, const void *gcache, d
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_finalize, dim3(1), dim3(FIN_THREADS), 0, s, partials, g.nparts, out);
+    return hipGetLastError();
+}
+
+template <typename S, typename C>
+static hipError_t diag_diss_t(const Lattice &L, int cur, const void *gcache, double svisc,
+                              double *partials, double *out, hipStream_t s) {
+    const RedGeo g = make_red_geo<S>(L);
+    const C cte1 = (C)(L.shifted ? 1.0 : 0.0);
+    double *pd = partials, *ps = partials + (size_t)g.nparts * RED_NV;
+    if (gcache)
+        hipLaunchKernelGGL((k_diag_diss<S, C, 1>), dim3(g.nparts), dim3(Tile<S>::X, Tile<S>::Y),
+                           0, s, pops_of<S>(L, cur), (const S *)gcache, g, cte1, (C)svisc, pd, ps);
+    else
+        hipLaunchKernelGGL((k_diag_diss<S, C, 0>), dim3(g.nparts), dim3(Tile<S>::X, Tile<S>::Y),
+                           0, s, pops_of<S>(L, cur), (const S *)nullptr, g, cte1, (C)svisc, pd,
+                           ps);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(FIN_THREADS), 0, s, pd, g.nparts, out);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(FIN_THREADS), 0, s, ps, g.nparts, out + RED_NV);
     return hipGetLastError();
 }
 
@@ -672,6 +703,10 @@ hipError_t launch_diagno(const Lattice &L, int cur, double *partials, double *ou
 hipError_t launch_dissipation(const Lattice &L, int cur, const void *gcache, double svisc,
                               double *partials, double *out, hipStream_t s) {
     BGK_DISPATCH(L.storage, dissipation_t, L, cur, gcache, svisc, partials, out, s);
+}
+hipError_t launch_diag_diss(const Lattice &L, int cur, const void *gcache, double svisc,
+                            double *partials, double *out, hipStream_t s) {
+    BGK_DISPATCH(L.storage, diag_diss_t, L, cur, gcache, svisc, partials, out, s);
 }
 hipError_t launch_pack_planes(const Lattice &L, void *const *pops, int top, int bot, void *send,
                               hipStream_t s) {

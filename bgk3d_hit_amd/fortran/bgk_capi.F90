@@ -137,6 +137,17 @@ module bgk_capi
        real(c_double), intent(out) :: out(2)
      end function bgk_dissipation
 
+     ! diagno and dissipation in one pass: out(1:5) as bgk_diagno, out(6:7)
+     ! as bgk_dissipation (bit-identical to the two calls)
+     integer(c_int) function bgk_diagnostics(ctx, svisc, mode, out) &
+          bind(c, name='bgk_diagnostics')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), value :: svisc
+       integer(c_int), value :: mode
+       real(c_double), intent(out) :: out(7)
+     end function bgk_diagnostics
+
      integer(c_int) function bgk_get_box(ctx, x0, x1, y0, y1, z0, z1, out) &
           bind(c, name='bgk_get_box')
        import :: c_int, c_ptr, c_double

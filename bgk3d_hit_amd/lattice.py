@@ -163,6 +163,14 @@ class Lattice:
                                           else L.DIAG_CONSISTENT, _dptr(out)))
         return out
 
+    def diagnostics(self, svisc, refcompat=False):
+        """diagno() and dissipation(svisc) in one pass over the populations
+        (bit-identical to the two calls): returns (5 values, 2 values)."""
+        out = np.zeros(7)
+        L.check(self._lib.bgk_diagnostics(self._h, float(svisc), L.DIAG_REFCOMPAT if refcompat
+                                          else L.DIAG_CONSISTENT, _dptr(out)))
+        return out[:5].copy(), out[5:].copy()
+
     def get_box(self, x0, x1, y0, y1, z0, z1):
         """Populations of a sub-box, shape (19, z1-z0, y1-y0, x1-x0)."""
         out = np.empty((19, z1 - z0, y1 - y0, x1 - x0))

@@ -155,6 +155,13 @@ int bgk_diagno(bgk_ctx *ctx, int mode, double out[5]);
  * reference's loctot(1..2), before its output normalisation).              */
 int bgk_dissipation(bgk_ctx *ctx, double svisc, int mode, double out[2]);
 
+/* diagno and dissipation of the current state in one pass over the
+ * populations: out[0..4] = bgk_diagno's out, out[5..6] = bgk_dissipation's,
+ * bit-identical to the two calls.  Replaces the pair the reference's
+ * diagnostic makes back to back (SRC/diagnostic.F90:47-49); no single
+ * reference routine does both.                                              */
+int bgk_diagnostics(bgk_ctx *ctx, double svisc, int mode, double out[7]);
+
 /* Populations of the box [x0,x1) x [y0,y1) x [z0,z1) (0-based, interior
  * sites) into out: 19 planes of (x1-x0)(y1-y0)(z1-z0) doubles, x fastest.
  * Device-side gather replacing the host-array reads of probe_global

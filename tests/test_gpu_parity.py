@@ -429,3 +429,26 @@ def test_get_box_matches_full_download(gpu):
         assert np.array_equal(g.get_box(x0, x1, y0, y1, z0, z1), f[:, z0:z1, y0:y1, x0:x1])
     with pytest.raises(bgk.BGKError):
         g.get_box(0, 41, 0, 1, 0, 1)
+
+
+@pytest.mark.parametrize("kind", ["f64", "f32", "f32mix", "f16", "bf16"])
+@pytest.mark.parametrize("layout,refcompat", [("periodic", False), ("periodic", True),
+                                              ("slab", False), ("slab", True)])
+def test_fused_diagnostics_equal_separate_calls(gpu, kind, layout, refcompat):
+    """bgk_diagnostics (diagno + dissipation in one pass) returns exactly what
+    bgk_diagno and bgk_dissipation return, in every mode and layout."""
+    say("fused diagnostics", kind, layout, refcompat)
+    shape = (36, 22, 19)
+    kw = dict(omega=1.93, storage=kind, layout=layout)
+    if layout == "periodic":
+        kw["refcompat_halos"] = refcompat
+    g = bgk.Lattice(*shape, **kw)
+    g.init_hit(4.0, 0.03, 77)
+    nu = bgk.svisc_from_omega(1.93)
+    for n in (0, 1, 6):
+        g.step(n)
+        d, s = g.diagnostics(nu, refcompat=refcompat)
+        d2 = g.diagno(refcompat=refcompat)
+        s2 = g.dissipation(nu, refcompat=refcompat)
+        assert np.array_equal(d, d2), (n, d, d2)
+        assert np.array_equal(s, s2), (n, s, s2)
