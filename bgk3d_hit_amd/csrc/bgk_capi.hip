@@ -382,6 +382,7 @@ int bgk_create(int lx, int ly, int lz, int storage, int layout, int flags, doubl
     L.ystride = 19LL * lx;
     L.zstride = 19LL * lx * ly;
     L.copy_elems = 19LL * plane;
+    L.wide_offsets = (flags & BGK_FLAG_WIDE_OFFSETS) ? 1 : 0;
     c->flags = flags;
     c->omega = omega;
     c->timing.on = (flags & BGK_FLAG_TIMING) != 0;

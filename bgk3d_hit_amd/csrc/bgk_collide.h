@@ -180,7 +180,7 @@ __device__ __forceinline__ void buf_st(S v, __amdgpu_buffer_rsrc_t r, uint32_t v
 // whether the in-plane byte offsets (x + y ystride, plus the population
 // offsets of the slab plane buffer's pointers) fit 32 bits
 inline bool offsets_fit_32(const Lattice &L, size_t elem_bytes) {
-    return (unsigned long long)L.zstride * elem_bytes <= (1ULL << 32);
+    return !L.wide_offsets && (unsigned long long)L.zstride * elem_bytes <= (1ULL << 32);
 }
 
 // ------------------------------------------------------------------ collide

@@ -47,6 +47,7 @@ struct Lattice {
     int two_step;          // bgk_step may fuse two time steps per pass: 0 no,
                            // 1 where it pays, 2 at any size
     int ncu;               // compute units of the device
+    int wide_offsets;      // BGK_FLAG_WIDE_OFFSETS: never take 32-bit offsets
 };
 
 // Element offset of interior site i (x fastest, then y, then z) of a

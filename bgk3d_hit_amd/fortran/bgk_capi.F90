@@ -15,7 +15,8 @@ module bgk_capi
        BGK_STORE_F32_MIXED = 2, BGK_STORE_F16 = 3, BGK_STORE_BF16 = 4
   integer(c_int), parameter :: BGK_LAYOUT_PERIODIC = 0, BGK_LAYOUT_SLAB_Z = 1
   integer(c_int), parameter :: BGK_FLAG_NOSHIFT = 1, BGK_FLAG_REFCOMPAT_HALOS = 2, &
-       BGK_FLAG_TIMING = 4
+       BGK_FLAG_TIMING = 4, BGK_FLAG_SINGLE_STEP = 8, BGK_FLAG_TWO_STEP_ANY_SIZE = 16, &
+       BGK_FLAG_WIDE_OFFSETS = 32
   integer(c_int), parameter :: BGK_DIAG_CONSISTENT = 0, BGK_DIAG_REFCOMPAT = 1
 
   interface

@@ -55,14 +55,15 @@ class Lattice:
     fuses at any size (testing)."""
 
     def __init__(self, lx, ly, lz, omega, storage="f64", layout="periodic", shifted=True,
-                 refcompat_halos=False, timing=False, two_step=True):
+                 refcompat_halos=False, timing=False, two_step=True, wide_offsets=False):
         self._lib = L.load()
         kind = L.STORAGE_NAMES[storage] if isinstance(storage, str) else int(storage)
         lay = {"periodic": L.LAYOUT_PERIODIC, "slab": L.LAYOUT_SLAB_Z}[layout]
         flags = (0 if shifted else L.FLAG_NOSHIFT) | \
                 (L.FLAG_REFCOMPAT_HALOS if refcompat_halos else 0) | \
                 (L.FLAG_TIMING if timing else 0) | \
-                {True: 0, False: L.FLAG_SINGLE_STEP, "always": L.FLAG_TWO_STEP_ANY_SIZE}[two_step]
+                {True: 0, False: L.FLAG_SINGLE_STEP, "always": L.FLAG_TWO_STEP_ANY_SIZE}[two_step] | \
+                (L.FLAG_WIDE_OFFSETS if wide_offsets else 0)
         h = ctypes.c_void_p()
         L.check(self._lib.bgk_create(lx, ly, lz, kind, lay, flags, float(omega), ctypes.byref(h)))
         self._h = h

@@ -60,6 +60,8 @@ typedef struct bgk_ctx bgk_ctx;
                                       time step (no two-step fusion)               */
 #define BGK_FLAG_TWO_STEP_ANY_SIZE 16 /* bgk_step: fuse two steps per pass also on
                                       lattices too small for it to pay (tests)     */
+#define BGK_FLAG_WIDE_OFFSETS 32   /* 64-bit in-plane offsets in every kernel, the
+                                      path of x-y planes above 4 GiB (tests)       */
 
 /* diagnostic modes */
 #define BGK_DIAG_CONSISTENT 0 /* gathers read the current state (fresh periodic images) */

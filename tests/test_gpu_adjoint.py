@@ -51,6 +51,21 @@ def test_adjoint_two_step_passes_equal_single_steps(gpu, kind, levels, K):
     assert np.array_equal(ga, gb)
 
 
+@pytest.mark.parametrize("kind", ["f64", "f32", "f32mix"])
+def test_adjoint_wide_offsets_equal(gpu, kind):
+    """The 64-bit-offset kernels of the adjoint (taped forward, two-step
+    adjoint sweep, forced by BGK_FLAG_WIDE_OFFSETS) give the same gradient."""
+    shape, omega, T = (70, 9, 20), 1.9, 13
+    say("adjoint wide offsets", kind)
+    u0 = bgk.hit_velocity(*shape, 2.0, 0.03, 8)
+    a = bgk.Lattice(*shape, omega=omega, storage=kind, two_step="always")
+    b = bgk.Lattice(*shape, omega=omega, storage=kind, two_step="always", wide_offsets=True)
+    Ja, ga, _ = a.adjoint_ke(u0, T, ckpt_every=5, levels=1)
+    Jb, gb, _ = b.adjoint_ke(u0, T, ckpt_every=5, levels=1)
+    assert Ja == Jb
+    assert np.array_equal(ga, gb)
+
+
 def test_adjoint_checkpoint_levels_agree_bitwise(gpu):
     shape, omega, T = (24, 24, 24), 1.9, 30
     say("adjoint levels bitwise")

@@ -305,11 +305,11 @@ def test_two_step_pass_512cubed_fp32_checksum(gpu):
     assert b.checksum() == ca
 
 
-def test_two_step_pass_64bit_offsets(gpu):
-    """A lattice whose population arrays exceed 4 GiB (1040^3 FP32, 4.5 GB per
-    array) takes the 64-bit-offset kernels; two-step passes == single steps."""
+def test_two_step_pass_1040cubed(gpu):
+    """1040^3 FP32 (84 GB of populations, tiles not dividing the lattice):
+    two-step passes == single steps."""
     n = 1040
-    say("two-step 1040^3 f32, 64-bit offsets")
+    say("two-step 1040^3 f32")
     a = bgk.Lattice(n, n, n, omega=1.95, storage="f32", timing=True)
     a.init_hit(4.0, 0.02, 77)
     a.step(4)
@@ -452,3 +452,25 @@ def test_fused_diagnostics_equal_separate_calls(gpu, kind, layout, refcompat):
         s2 = g.dissipation(nu, refcompat=refcompat)
         assert np.array_equal(d, d2), (n, d, d2)
         assert np.array_equal(s, s2), (n, s, s2)
+
+
+@pytest.mark.parametrize("kind", ["f64", "f32", "f32mix", "f16", "bf16"])
+@pytest.mark.parametrize("layout", ["periodic", "slab"])
+def test_wide_offsets_equal_32bit_offsets(gpu, kind, layout):
+    """The 64-bit-offset kernels (taken for x-y planes above 4 GiB, forced
+    here by BGK_FLAG_WIDE_OFFSETS) give the same populations as the 32-bit
+    ones, for single steps and two-step passes."""
+    say("wide offsets", kind, layout)
+    shape = (40, 26, 21)
+    u = bgk.hit_velocity(*shape, 4.0, 0.03, 19)
+    for two_step in ("always", False):
+        a = bgk.Lattice(*shape, omega=1.93, storage=kind, layout=layout, two_step=two_step)
+        b = bgk.Lattice(*shape, omega=1.93, storage=kind, layout=layout, two_step=two_step,
+                        wide_offsets=True)
+        a.init_equilibrium(u)
+        b.init_equilibrium(u)
+        a.step(7)
+        b.step(7)
+        assert np.array_equal(a.get_populations(), b.get_populations()), two_step
+        a.close()
+        b.close()
