@@ -95,16 +95,8 @@ __device__ __forceinline__ int xcd_remap(int b, int G) {
 template <bool O32>
 using off_t_ = typename std::conditional<O32, uint32_t, size_t>::type;
 
-template <typename S, typename O>
-__device__ __forceinline__ S ld_at(const S *base, O off) {
-    return *(const S *)((const char *)base + off);
-}
-template <typename S, typename O>
-__device__ __forceinline__ void st_at(S *base, O off, S v) {
-    *(S *)((char *)base + off) = v;
-}
-// the same through a global-address-space base (plane_ptr); the element
-// moves as an unsigned integer of its size
+// The base is a global-address-space pointer (plane_ptr); the element moves
+// as an unsigned integer of its size.
 template <int N>
 struct uint_of;
 template <>
