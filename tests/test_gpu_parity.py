@@ -268,6 +268,15 @@ def test_timing_counters(gpu, two_step, launches):
     assert t["collide_ms"] > 0 and t["step_ms"] >= t["collide_ms"] * 0.5
 
 
+@pytest.mark.parametrize("n,launches", [(64, 20), (128, 10), (256, 10)])
+def test_two_step_engages_where_it_pays(gpu, n, launches):
+    """bgk_step's chunk model: 64^3 (104 workgroups, fewer than half the
+    CUs) takes single steps; 128^3 and up take two steps per pass."""
+    g = bgk.Lattice(n, n, n, omega=1.9, storage="f32", timing=True)
+    g.step(20)
+    assert g.timing()["collide_launches"] == launches
+
+
 @pytest.mark.parametrize("kind", ["f64", "f32", "f32mix", "f16", "bf16"])
 @pytest.mark.parametrize("shape", [(70, 9, 6), (64, 16, 130), (24, 20, 16), (5, 3, 1),
                                    (130, 17, 65), (1, 1, 2)])
