@@ -63,6 +63,26 @@ __global__ __launch_bounds__(256) void k_lambda_final(Pops<S> P, SiteMap m, C *M
     }
 }
 
+// a + c v for c in {-1, 0, +1} known at compile time: the c = 0 term is
+// left out and c = +-1 is an add or subtract (without fast-math the
+// compiler must keep a multiplication by 0 and the addition of its signed
+// zero; leaving them out changes at most the sign of a zero sum)
+template <typename C>
+__device__ __forceinline__ C add_c(C a, int c, C v) {
+    return c == 0 ? a : (c > 0 ? a + v : a - v);
+}
+// c_q . u over the nonzero components of c_q
+template <typename C>
+__device__ __forceinline__ C dot_c(int q, C ux, C uy, C uz) {
+    const int n = (cx(q) != 0) + (cy(q) != 0) + (cz(q) != 0);
+    if (n == 0) return (C)0;
+    C r = cx(q) != 0 ? (cx(q) > 0 ? ux : -ux) : (cy(q) != 0 ? (cy(q) > 0 ? uy : -uy)
+                                                            : (cz(q) > 0 ? uz : -uz));
+    if (cx(q) != 0 && cy(q) != 0) r = add_c(r, cy(q), uy);
+    if ((cx(q) != 0 || cy(q) != 0) && cz(q) != 0) r = add_c(r, cz(q), uz);
+    return r;
+}
+
 // mu_k = (1 - omega) lambda_k + omega [A + (B - 3 S u) . c_k] at one site
 template <typename C>
 __device__ __forceinline__ void adjoint_site(const C (&lam)[19], C ux, C uy, C uz, C omega,
@@ -72,19 +92,19 @@ __device__ __forceinline__ void adjoint_site(const C (&lam)[19], C ux, C uy, C u
 #pragma unroll
     for (int q = 0; q < 19; q++) {
         const C wl = (C)wq(q) * lam[q];
-        const C cu = (C)cx(q) * ux + (C)cy(q) * uy + (C)cz(q) * uz;
+        const C cu = dot_c<C>(q, ux, uy, uz);
         A += wl * ((C)1 - (C)4.5 * cu * cu + (C)1.5 * usq);
         const C s = wl * ((C)3 + (C)9 * cu);
-        Bx += s * (C)cx(q);
-        By += s * (C)cy(q);
-        Bz += s * (C)cz(q);
+        Bx = add_c(Bx, cx(q), s);
+        By = add_c(By, cy(q), s);
+        Bz = add_c(Bz, cz(q), s);
         Sw += wl;
     }
     const C Gx = Bx - (C)3 * Sw * ux, Gy = By - (C)3 * Sw * uy, Gz = Bz - (C)3 * Sw * uz;
+    const C om1 = (C)1 - omega;
 #pragma unroll
     for (int q = 0; q < 19; q++)
-        mu[q] = ((C)1 - omega) * lam[q] +
-                omega * (A + Gx * (C)cx(q) + Gy * (C)cy(q) + Gz * (C)cz(q));
+        mu[q] = om1 * lam[q] + omega * add_c(add_c(add_c(A, cx(q), Gx), cy(q), Gy), cz(q), Gz);
 }
 
 // one adjoint step: lambda^{t+1}_i(x) = M^{t+1}_i(x + c_i) (shifted loads),
