@@ -150,6 +150,7 @@ __global__ __launch_bounds__(256) void k_adjoint_step(const C *__restrict__ Min,
 template <typename C>
 struct AdjointOp {
     static constexpr int SGN = -1;
+    static constexpr bool PINGPONG = true;
     struct Extra {
         C u[3];
     };

@@ -385,6 +385,9 @@ __device__ __forceinline__ void lds_barrier() {
 template <typename C, bool TAPE, bool SH = false>
 struct CollideOp {
     static constexpr int SGN = 1;
+    // two load register sets (see two_step_pass); the taping variant keeps
+    // one, its tape pointers would not fit beside the second set
+    static constexpr bool PINGPONG = !TAPE;
     struct Extra {};
     C omega, cte1;
     C *tape0, *tape1;
@@ -617,11 +620,11 @@ __device__ __forceinline__ void two_step_pass(const Pops<S> &src, const Pops<S> 
     S ra[19];
     X_t xa;
     load_plane(za - 1, ra, xa);
-    if constexpr (O32 && !FACES) {
+    if constexpr (O32 && !FACES && Op::PINGPONG) {
         // two register sets taking turns: the loads of plane z+1 land in the
         // set plane z-1 used, so the compute copy of a plane costs no moves
-        // (not for the 64-bit-offset and face-writing variants, where the
-        // second set would spill)
+        // (not for the 64-bit-offset, face-writing and taping variants,
+        // where the second set would spill)
         S rb[19];
         X_t xb;
         int it = 0;
