@@ -86,6 +86,21 @@ __device__ __forceinline__ int xcd_remap(int b, int G) {
     return (xcd < r) ? xcd * (q + 1) + slot : r * (q + 1) + (xcd - r) * q + slot;
 }
 
+// Block order of the two-step pass, where each workgroup fills a CU: the
+// workgroups that run at the same time (one round: gsz per XCD on 8 XCDs)
+// take 8*gsz consecutive tiles, XCD k the k-th group of gsz.  The groups of
+// one round are neighbours in y on different XCDs and fetch the halo rows
+// they share at the same time; the next round continues with the next 8*gsz
+// tiles.  Workgroups past the last full round keep their index.  Measured
+// against xcd_remap (each XCD one contiguous eighth of the launch): 2.5 %
+// faster in FP32, 1 % in FP16 at 512^3.
+__device__ __forceinline__ int xcd_round_remap(int b, int G, int gsz) {
+    const int full = (G / (8 * gsz)) * 8 * gsz;
+    if (b >= full) return b;
+    const int k = b & 7, slot = b >> 3;
+    return ((slot / gsz) * 8 + k) * gsz + slot % gsz;
+}
+
 // Element access by byte offset from a per-population base pointer.  The
 // kernels add a site's z plane to the base (uniform across the workgroup, so
 // the sum stays in scalar registers) and pass the in-plane part as the
@@ -336,6 +351,7 @@ struct Geo2 {
     int zlo, zhi, zoff, periodic_z;
     long long ystride, zstride;  // element pitches of rows and planes (as Geo)
     unsigned pstride;            // BUF: bytes from population q to q+1
+    int xcd_group;               // workgroups per XCD in one round (CUs / 8)
     void *faces;  // FACES: face images of the first step (k_pack_faces layout)
 };
 
@@ -355,6 +371,7 @@ static Geo2 make_geo2(const Lattice &L, int tx, int ty, int zc, int *nblocks) {
     g.ystride = L.ystride;
     g.zstride = L.zstride;
     g.pstride = (unsigned)((const char *)L.pop[0][1] - (const char *)L.pop[0][0]);
+    g.xcd_group = L.ncu >= 8 ? L.ncu / 8 : 1;
     g.faces = nullptr;
     *nblocks = g.tiles_x * g.tiles_y * ((g.zhi - g.zlo + g.zc - 1) / g.zc);
     return g;
@@ -439,7 +456,7 @@ __device__ __forceinline__ void two_step_pass(const Pops<S> &src, const Pops<S> 
     // planes before, QR the rest population
     constexpr int QN = SGN > 0 ? 14 : 5, QO = SGN > 0 ? 5 : 14, QR = 18;
 
-    const int t = xcd_remap((int)blockIdx.x, (int)gridDim.x);
+    const int t = xcd_round_remap((int)blockIdx.x, (int)gridDim.x, g.xcd_group);
     const int tx = t % g.tiles_x;
     const int r = t / g.tiles_x;
     const int ty = r % g.tiles_y;
