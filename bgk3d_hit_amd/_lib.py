@@ -21,6 +21,7 @@ LAYOUT_PERIODIC, LAYOUT_SLAB_Z = 0, 1
 FLAG_NOSHIFT, FLAG_REFCOMPAT_HALOS, FLAG_TIMING, FLAG_SINGLE_STEP = 1, 2, 4, 8
 FLAG_TWO_STEP_ANY_SIZE = 16
 FLAG_WIDE_OFFSETS = 32
+FLAG_NO_THREE_STEP = 64
 # diagnostic modes
 DIAG_CONSISTENT, DIAG_REFCOMPAT = 0, 1
 

@@ -176,6 +176,15 @@ int step_periodic(bgk_ctx *c, int nsteps, bool count) {
     std::vector<Box> all{full_box(c->lat)};
     const bool images = (c->flags & BGK_FLAG_REFCOMPAT_HALOS) != 0;
     const int n0 = nsteps;
+    if (!images && three_step_ok(c->lat)) {
+        for (; nsteps >= 3; nsteps -= 3) {
+            hipEvent_t ev;
+            BGK_HIP(timing_begin(c, TCLS_COLLIDE, c->s0, &ev));
+            BGK_HIP(launch_collide3(c->lat, c->cur, c->omega, c->s0));
+            BGK_HIP(timing_end(c, TCLS_COLLIDE, c->s0, ev));
+            c->cur ^= 1;
+        }
+    }
     if (two_step_ok(c->lat)) {
         for (; nsteps >= 2; nsteps -= 2) {
             const int a = c->cur;
@@ -397,6 +406,7 @@ int bgk_create(int lx, int ly, int lz, int storage, int layout, int flags, doubl
     L.two_step = !(flags & BGK_FLAG_SINGLE_STEP)
                      ? ((flags & BGK_FLAG_TWO_STEP_ANY_SIZE) ? 2 : 1)
                      : 0;
+    L.three_step = (flags & BGK_FLAG_NO_THREE_STEP) ? 0 : 1;
     // two copies in the interleaved [z][y][q][x] layout (see Lattice)
     hipError_t e = hipMalloc(&c->pop_alloc, cbytes * 2);
     if (e != hipSuccess) return bail(hip_fail(e, "hipMalloc(populations)"));

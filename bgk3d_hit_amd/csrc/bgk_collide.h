@@ -663,4 +663,212 @@ __global__ __launch_bounds__(collide2_threads(TX, TY), MINB) void k_collide2(
     two_step_pass<S, C, O32, TX, TY, CollideOp<C, TAPE, SH>, FACES, O32>(src, dst, g, op, lds);
 }
 
+// ------------------------------------------------------ three steps per pass
+// The two-step pass one level deeper: a workgroup sweeps z once and advances
+// its tile three time steps, the first two steps' states kept on chip.  Per
+// plane iteration z: S1(z) at every site of the (TX+4) x (TY+4) grown tile,
+// S2(z-1) at the (TX+2) x (TY+2) middle sites (pulled from S1 of planes z,
+// z-1, z-2), S3(z-2) at the TX x TY own sites (pulled from S2 likewise).  The
+// x/y-moving populations of S1 and of S2 are shared through LDS in the
+// group/slot layout of the two-step pass (20 planes each); the z-only ones
+// stay in the column's registers.  Every intermediate value is rounded to the
+// storage type exactly as a store and reload would round it, so a pass equals
+// three single steps bit for bit.  Periodic lattice, 32-bit offsets through
+// one buffer resource per z plane (populations g.pstride bytes apart).
+//
+// Cost against the two-step pass: a chunk of zc planes takes zc + 4 plane
+// iterations (two-step: zc + 2) and three collisions per site become
+// (TX+4)(TY+4)/(TX TY) + (TX+2)(TY+2)/(TX TY) + 1 (64 x 10: 3.73, i.e. 1.24
+// per step, against 1.21 per step of the two-step pass's 128 x 5 tiles),
+// while the populations cross HBM once per three steps instead of two.
+template <typename S>
+struct Tile3 {
+    static constexpr int X = 64, Y = 10, ZC = 64;
+};
+
+// threads: one per site of the TX-wide x range of the TY+4 grown rows, plus
+// one wave for the four grown columns (4 (TY+4) <= 64)
+constexpr int collide3_threads(int tx, int ty) { return tx * (ty + 4) + 64; }
+
+// Planes per workgroup of the three-step pass: as two_step_zc, with zc + 4
+// plane iterations per chunk.  0 when the launch would not reach half of the
+// CUs (unless L.two_step == 2) or the lattice is too thin in z.
+inline int three_step_zc(const Lattice &L, int tx, int ty, int zc_max) {
+    const int nz = L.nz;
+    if (L.layout != BGK_LAYOUT_PERIODIC || nz < 4) return 0;
+    const long long txy = (long long)((L.nx + tx - 1) / tx) * ((L.ny + ty - 1) / ty);
+    const long long ncu = L.ncu > 0 ? L.ncu : 1;
+    const int lo = nz < TWO_STEP_MIN_ZC ? nz : TWO_STEP_MIN_ZC;
+    const int hi = nz < zc_max ? nz : zc_max;
+    int best = lo;
+    long long best_cost = -1;
+    for (int zc = lo; zc <= hi; zc++) {
+        const long long nb = txy * ((nz + zc - 1) / zc);
+        const long long cost = ((nb + ncu - 1) / ncu) * (zc + 4);
+        if (best_cost < 0 || cost <= best_cost) {
+            best_cost = cost;
+            best = zc;
+        }
+    }
+    const long long nb = txy * ((nz + best - 1) / best);
+    if (L.two_step != 2 && 2 * nb < ncu) return 0;
+    return best;
+}
+
+template <typename S, typename C, int TX, int TY, bool SH>
+__global__ __launch_bounds__(collide3_threads(TX, TY), 1) void k_collide3(Pops<S> src,
+                                                                          Pops<S> dst, Geo2 g,
+                                                                          C omega, C cte1) {
+    constexpr int EX = TX + 4, EY = TY + 4;
+    constexpr int NMAIN = TX * EY;
+    static_assert(4 * EY <= 64, "the edge wave covers four grown columns");
+    constexpr uint32_t es = sizeof(S);
+    constexpr int QN = 14, QO = 5, QR = 18;  // z-only: newest plane, two before, rest
+    __shared__ S l1[20][EY][EX];             // S1, site (ey, ex)
+    __shared__ S l2[20][EY - 2][EX - 2];     // S2, site (ey, ex) at [ey-1][ex-1]
+
+    const int t = xcd_round_remap((int)blockIdx.x, (int)gridDim.x, g.xcd_group);
+    const int tx = t % g.tiles_x;
+    const int r = t / g.tiles_x;
+    const int ty = r % g.tiles_y;
+    const int za = (r / g.tiles_y) * g.zc;
+    const int zb = min(g.nz, za + g.zc);
+
+    const int tid = (int)threadIdx.x;
+    int ex, ey;
+    bool active;
+    if (tid < NMAIN) {
+        ex = tid % TX + 2;
+        ey = tid / TX;
+        active = true;
+    } else {
+        const int e = tid - NMAIN;
+        const int col = e / EY;  // grown columns 0, 1, EX-2, EX-1
+        active = e < 4 * EY;
+        ex = col < 2 ? col : EX - 4 + col;
+        ey = e % EY;
+    }
+    const int lx = tx * TX + ex - 2, ly = ty * TY + ey - 2;  // may leave the box
+    const int gx = ((lx % g.nx) + g.nx) % g.nx;
+    const int gy = ((ly % g.ny) + g.ny) % g.ny;
+    const bool mid = active && ex >= 1 && ex <= EX - 2 && ey >= 1 && ey <= EY - 2;
+    const bool owner = tid < NMAIN && ey >= 2 && ey <= EY - 3 && lx < g.nx && ly < g.ny;
+
+    const int xm = (gx == 0) ? g.nx - 1 : gx - 1;
+    const int xp = (gx == g.nx - 1) ? 0 : gx + 1;
+    const int ym = (gy == 0) ? g.ny - 1 : gy - 1;
+    const int yp = (gy == g.ny - 1) ? 0 : gy + 1;
+    const uint32_t X[3] = {(uint32_t)xm * es, (uint32_t)gx * es, (uint32_t)xp * es};
+    const uint32_t ys = (uint32_t)g.ystride * es;
+    const uint32_t Y[3] = {(uint32_t)ym * ys, (uint32_t)gy * ys, (uint32_t)yp * ys};
+    const ptrdiff_t plane = (ptrdiff_t)g.zstride;
+
+    auto wrapz = [&](int z) { return z < 0 ? z + g.nz : (z >= g.nz ? z - g.nz : z); };
+    auto load_plane = [&](int z, S(&raw)[19]) {
+        const int p = wrapz(z);
+        const int zm = (p == 0) ? g.nz - 1 : p - 1;
+        const int zp = (p == g.nz - 1) ? 0 : p + 1;
+        const S *b0 = src.p[0];
+        const __amdgpu_buffer_rsrc_t R[3] = {raw_rsrc(b0 + (ptrdiff_t)zm * plane),
+                                             raw_rsrc(b0 + (ptrdiff_t)p * plane),
+                                             raw_rsrc(b0 + (ptrdiff_t)zp * plane)};
+#pragma unroll
+        for (int q = 0; q < 19; q++)
+            raw[q] = buf_ld<S>(R[1 - cz(q)], X[1 - cx(q)] + Y[1 - cy(q)], (uint32_t)q * g.pstride);
+    };
+    // pulled value of population q for the site (sy, sx) of a state whose
+    // x/y-moving populations are in `lds` (newest plane group 0, slot pb of
+    // the plane two before); the z-only ones from the column's registers
+    auto pull = [&](auto &lds, int q, int sy, int sx, int pb, S newest, S older, S rest) {
+        const int yy = sy - cy(q), xx = sx - cx(q);
+        if (pgroup<1>(q) == 0) return lds[pbase(0) + pslot<1>(q)][yy][xx];
+        if (pgroup<1>(q) == 1) return lds[pbase(1) + pslot<1>(q)][yy][xx];
+        if (pgroup<1>(q) == 2) return lds[pb + pslot<1>(q)][yy][xx];
+        return q == QO ? older : (q == QN ? newest : rest);
+    };
+
+    S r1O_1 = S{}, r1O_2 = S{}, r1R_1 = S{};  // S1 of this column at planes z-1, z-2
+    S r2O_1 = S{}, r2O_2 = S{}, r2R_1 = S{};  // S2 of this column, one plane behind
+    const int nit = zb - za + 4;              // iterations z = za-2 .. zb+1
+    auto iteration = [&](int it, S(&cur)[19], S(&nxt)[19]) {
+        const int z = za - 2 + it;
+        const int pb = pbase(2) + 4 * (it & 1);  // slot of S(z-2), then of S(z)
+        S s1[19], s2[19];
+        C f[19];
+#pragma unroll
+        for (int q = 0; q < 19; q++) {
+            f[q] = Storage<S>::template load<C>(cur[q]);
+            asm volatile("" : "+v"(f[q]));  // see two_step_pass
+        }
+        if (it + 1 < nit) load_plane(z + 1, nxt);
+        if (active) {  // S1 at plane z
+            C b[19], ux, uy, uz;
+            collide_site<C, SH>(f, b, omega, cte1, ux, uy, uz);
+#pragma unroll
+            for (int q = 0; q < 19; q++) s1[q] = Storage<S>::template store<C>(b[q]);
+#pragma unroll
+            for (int q = 0; q < 19; q++)
+                if (pgroup<1>(q) == 0) l1[pbase(0) + pslot<1>(q)][ey][ex] = s1[q];
+        }
+        lds_barrier();
+        const bool do2 = mid && it >= 2;
+        if (do2) {  // S2 at plane z-1
+            C f2[19], b[19], ux, uy, uz;
+#pragma unroll
+            for (int q = 0; q < 19; q++)
+                f2[q] = Storage<S>::template load<C>(pull(l1, q, ey, ex, pb, s1[QN], r1O_2, r1R_1));
+            collide_site<C, SH>(f2, b, omega, cte1, ux, uy, uz);
+#pragma unroll
+            for (int q = 0; q < 19; q++) s2[q] = Storage<S>::template store<C>(b[q]);
+#pragma unroll
+            for (int q = 0; q < 19; q++)
+                if (pgroup<1>(q) == 0) l2[pbase(0) + pslot<1>(q)][ey - 1][ex - 1] = s2[q];
+        }
+        lds_barrier();
+        if (active) {
+#pragma unroll
+            for (int q = 0; q < 19; q++) {
+                if (pgroup<1>(q) == 1) l1[pbase(1) + pslot<1>(q)][ey][ex] = s1[q];
+                if (pgroup<1>(q) == 2) l1[pb + pslot<1>(q)][ey][ex] = s1[q];
+            }
+        }
+        if (owner && it >= 4) {  // S3 at plane z-2
+            C f3[19], b[19], ux, uy, uz;
+#pragma unroll
+            for (int q = 0; q < 19; q++)
+                f3[q] = Storage<S>::template load<C>(
+                    pull(l2, q, ey - 1, ex - 1, pb, s2[QN], r2O_2, r2R_1));
+            collide_site<C, SH>(f3, b, omega, cte1, ux, uy, uz);
+            const __amdgpu_buffer_rsrc_t R = raw_rsrc(dst.p[0] + (ptrdiff_t)wrapz(z - 2) * plane);
+#pragma unroll
+            for (int q = 0; q < 19; q++)
+                buf_st<S>(Storage<S>::template store<C>(b[q]), R, X[1] + Y[1],
+                          (uint32_t)q * g.pstride);
+        }
+        lds_barrier();
+        if (do2) {
+#pragma unroll
+            for (int q = 0; q < 19; q++) {
+                if (pgroup<1>(q) == 1) l2[pbase(1) + pslot<1>(q)][ey - 1][ex - 1] = s2[q];
+                if (pgroup<1>(q) == 2) l2[pb + pslot<1>(q)][ey - 1][ex - 1] = s2[q];
+            }
+            r2O_2 = r2O_1;
+            r2O_1 = s2[QO];
+            r2R_1 = s2[QR];
+        }
+        r1O_2 = r1O_1;
+        r1O_1 = s1[QO];
+        r1R_1 = s1[QR];
+    };
+
+    S ra[19], rb[19];
+    load_plane(za - 2, ra);
+    int it = 0;
+    for (; it + 1 < nit; it += 2) {  // two register sets taking turns
+        iteration(it, ra, rb);
+        iteration(it + 1, rb, ra);
+    }
+    if (it < nit) iteration(it, ra, rb);
+}
+
 }  // namespace bgk

@@ -46,6 +46,8 @@ struct Lattice {
     void *pop[2][19];      // population base pointers of copy 0/1
     int two_step;          // bgk_step may fuse two time steps per pass: 0 no,
                            // 1 where it pays, 2 at any size
+    int three_step;        // bgk_step may fuse three time steps per pass
+                           // (where two_step allows it)
     int ncu;               // compute units of the device
     int wide_offsets;      // BGK_FLAG_WIDE_OFFSETS: never take 32-bit offsets
 };
@@ -151,6 +153,10 @@ hipError_t launch_collide(const Lattice &L, int src, const Box &box, double omeg
 bool two_step_ok(const Lattice &L);
 hipError_t launch_collide2(const Lattice &L, int src, double omega, void *faces,
                            hipStream_t s);
+// three time steps in one pass from copy `src` into copy 1-src; only when
+// three_step_ok(L) (periodic layout, no face images)
+bool three_step_ok(const Lattice &L);
+hipError_t launch_collide3(const Lattice &L, int src, double omega, hipStream_t s);
 // diagno and dissipation in one pass: partials holds 2 * 8 * parts doubles,
 // out 16 (out[0..7] as launch_diagno's, out[8..15] as launch_dissipation's)
 hipError_t launch_diag_diss(const Lattice &L, int cur, const void *gcache, double svisc,

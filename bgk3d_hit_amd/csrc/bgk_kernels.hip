@@ -5,6 +5,7 @@
 //                      in the index math (or slab ghost planes in z)
 //   k_collide2         two time steps per pass (launchers here, kernel in
 //                      bgk_collide.h); on the slab for the bulk planes
+//   k_collide3         three time steps per pass (periodic, 4/2-byte storage)
 //   k_diagno           at-site moment sums (SRC/diagno.F90:73-115)
 //   k_dissipation      pulled Pi^neq / kinetic energy sums
 //                      (SRC/dissipation.F90:70-253)
@@ -72,6 +73,30 @@ static hipError_t collide2_t(const Lattice &L, int src, double omega, void *face
                              hipStream_t s) {
     return offsets_fit_32(L, sizeof(S)) ? collide2_o<S, C, true>(L, src, omega, faces, s)
                                         : collide2_o<S, C, false>(L, src, omega, faces, s);
+}
+
+// three steps from copy `src` into copy 1-src (periodic lattice, 32-bit
+// offsets, 4- or 2-byte storage computed in FP32)
+template <typename S, typename C, bool SH>
+static hipError_t collide3_sh(const Lattice &L, int src, double omega, hipStream_t s) {
+    using T = Tile3<S>;
+    const int zc = three_step_zc(L, T::X, T::Y, T::ZC);
+    if (zc <= 0) return hipErrorInvalidValue;  // caller checks three_step_ok first
+    int nblocks;
+    const Geo2 g = make_geo2(L, T::X, T::Y, zc, &nblocks);
+    hipLaunchKernelGGL((k_collide3<S, C, T::X, T::Y, SH>), dim3(nblocks),
+                       dim3(collide3_threads(T::X, T::Y)), 0, s, pops_of<S>(L, src),
+                       pops_of<S>(L, 1 - src), g, (C)omega, (C)(L.shifted ? 1.0 : 0.0));
+    return hipGetLastError();
+}
+template <typename S, typename C>
+static hipError_t collide3_t(const Lattice &L, int src, double omega, hipStream_t s) {
+    if constexpr (sizeof(S) == 8 || sizeof(C) == 8) {
+        return hipErrorInvalidValue;  // not offered (three_step_ok)
+    } else {
+        return L.shifted ? collide3_sh<S, C, true>(L, src, omega, s)
+                         : collide3_sh<S, C, false>(L, src, omega, s);
+    }
 }
 
 // --------------------------------------------------------------- reductions
@@ -691,6 +716,22 @@ bool two_step_ok(const Lattice &L) {
     case BGK_STORE_BF16: return two_step_ok_t<bf16_t, float>(L);
     default: return false;
     }
+}
+bool three_step_ok(const Lattice &L) {
+    if (!L.two_step || !L.three_step || L.layout != BGK_LAYOUT_PERIODIC) return false;
+    size_t es;
+    switch (L.storage) {
+    case BGK_STORE_F32: es = 4; break;
+    case BGK_STORE_F16:
+    case BGK_STORE_BF16: es = 2; break;
+    default: return false;
+    }
+    if (!offsets_fit_32(L, es)) return false;
+    return es == 4 ? three_step_zc(L, Tile3<float>::X, Tile3<float>::Y, Tile3<float>::ZC) > 0
+                   : three_step_zc(L, Tile3<half_t>::X, Tile3<half_t>::Y, Tile3<half_t>::ZC) > 0;
+}
+hipError_t launch_collide3(const Lattice &L, int src, double omega, hipStream_t s) {
+    BGK_DISPATCH(L.storage, collide3_t, L, src, omega, s);
 }
 hipError_t launch_collide2(const Lattice &L, int src, double omega, void *faces,
                            hipStream_t s) {

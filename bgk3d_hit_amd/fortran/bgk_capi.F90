@@ -16,7 +16,7 @@ module bgk_capi
   integer(c_int), parameter :: BGK_LAYOUT_PERIODIC = 0, BGK_LAYOUT_SLAB_Z = 1
   integer(c_int), parameter :: BGK_FLAG_NOSHIFT = 1, BGK_FLAG_REFCOMPAT_HALOS = 2, &
        BGK_FLAG_TIMING = 4, BGK_FLAG_SINGLE_STEP = 8, BGK_FLAG_TWO_STEP_ANY_SIZE = 16, &
-       BGK_FLAG_WIDE_OFFSETS = 32
+       BGK_FLAG_WIDE_OFFSETS = 32, BGK_FLAG_NO_THREE_STEP = 64
   integer(c_int), parameter :: BGK_DIAG_CONSISTENT = 0, BGK_DIAG_REFCOMPAT = 1
 
   interface

@@ -52,10 +52,13 @@ class Lattice:
 
     two_step: True fuses two time steps per pass in step() where the lattice
     is large enough for it to pay, False takes one pass per step, "always"
-    fuses at any size (testing)."""
+    fuses at any size (testing).  three_step: with two_step, also fuse three
+    time steps per pass (periodic layout, FP32/FP16/BF16 storage) where
+    two_step would fuse two."""
 
     def __init__(self, lx, ly, lz, omega, storage="f64", layout="periodic", shifted=True,
-                 refcompat_halos=False, timing=False, two_step=True, wide_offsets=False):
+                 refcompat_halos=False, timing=False, two_step=True, wide_offsets=False,
+                 three_step=True):
         self._lib = L.load()
         kind = L.STORAGE_NAMES[storage] if isinstance(storage, str) else int(storage)
         lay = {"periodic": L.LAYOUT_PERIODIC, "slab": L.LAYOUT_SLAB_Z}[layout]
@@ -63,7 +66,8 @@ class Lattice:
                 (L.FLAG_REFCOMPAT_HALOS if refcompat_halos else 0) | \
                 (L.FLAG_TIMING if timing else 0) | \
                 {True: 0, False: L.FLAG_SINGLE_STEP, "always": L.FLAG_TWO_STEP_ANY_SIZE}[two_step] | \
-                (L.FLAG_WIDE_OFFSETS if wide_offsets else 0)
+                (L.FLAG_WIDE_OFFSETS if wide_offsets else 0) | \
+                (0 if three_step else L.FLAG_NO_THREE_STEP)
         h = ctypes.c_void_p()
         L.check(self._lib.bgk_create(lx, ly, lz, kind, lay, flags, float(omega), ctypes.byref(h)))
         self._h = h

@@ -62,6 +62,8 @@ typedef struct bgk_ctx bgk_ctx;
                                       lattices too small for it to pay (tests)     */
 #define BGK_FLAG_WIDE_OFFSETS 32   /* 64-bit in-plane offsets in every kernel, the
                                       path of x-y planes above 4 GiB (tests)       */
+#define BGK_FLAG_NO_THREE_STEP 64  /* bgk_step: at most two time steps per pass
+                                      (no three-step fusion)                       */
 
 /* diagnostic modes */
 #define BGK_DIAG_CONSISTENT 0 /* gathers read the current state (fresh periodic images) */
@@ -136,8 +138,10 @@ int bgk_col_MC(bgk_ctx *ctx);
 int bgk_col_MC_masked(bgk_ctx *ctx, int opt);
 
 /* nsteps of the time loop body boundaries + col_MC (SRC/bgk3d_HIT.F90:127-146
- * without diagnostic).  Two steps per pass over the populations where the
- * lattice is large enough (unless BGK_FLAG_SINGLE_STEP), then single steps;
+ * without diagnostic).  Where the lattice is large enough (unless
+ * BGK_FLAG_SINGLE_STEP): three steps per pass over the populations (periodic
+ * layout, FP32/FP16/BF16 storage, no halo images; unless
+ * BGK_FLAG_NO_THREE_STEP), then two steps per pass, then single steps;
  * bit-identical to nsteps calls of bgk_col_MC, stale halo images included.  Slab layout: the interior collide overlaps the halo
  * pack/exchange/unpack and boundary collide on a second HIP stream (the
  * structure of SRC/bcond.comm.step9.F90:349 and

@@ -259,7 +259,8 @@ def test_refcompat_dissipation_needs_flag(gpu):
 
 @pytest.mark.parametrize("two_step,launches", [("always", 5), (True, 10), (False, 10)])
 def test_timing_counters(gpu, two_step, launches):
-    g = bgk.Lattice(64, 64, 64, omega=1.9, storage="f32", timing=True, two_step=two_step)
+    g = bgk.Lattice(64, 64, 64, omega=1.9, storage="f32", timing=True, two_step=two_step,
+                    three_step=False)
     g.init_hit(4.0, 0.02, 1)
     g.step(10)
     g.diagno()
@@ -272,7 +273,7 @@ def test_timing_counters(gpu, two_step, launches):
 def test_two_step_engages_where_it_pays(gpu, n, launches):
     """bgk_step's chunk model: 64^3 (104 workgroups, fewer than half the
     CUs) takes single steps; 128^3 and up take two steps per pass."""
-    g = bgk.Lattice(n, n, n, omega=1.9, storage="f32", timing=True)
+    g = bgk.Lattice(n, n, n, omega=1.9, storage="f32", timing=True, three_step=False)
     g.step(20)
     assert g.timing()["collide_launches"] == launches
 
@@ -287,7 +288,7 @@ def test_two_step_pass_equals_single_steps(gpu, kind, shape):
     and for odd step counts (a fused pass followed by one single step)."""
     say("two-step vs single", kind, shape)
     u = bgk.hit_velocity(*shape, 2.0, 0.03, 11)
-    a = bgk.Lattice(*shape, omega=1.93, storage=kind, two_step="always")
+    a = bgk.Lattice(*shape, omega=1.93, storage=kind, two_step="always", three_step=False)
     b = bgk.Lattice(*shape, omega=1.93, storage=kind, two_step=False)
     a.init_equilibrium(u)
     b.init_equilibrium(u)
@@ -302,7 +303,7 @@ def test_two_step_pass_512cubed_fp32_checksum(gpu):
     """The headline lattice: 512^3 FP32, 10 steps with the two-step pass ==
     10 single steps, bit for bit."""
     say("two-step 512^3 f32")
-    a = bgk.Lattice(512, 512, 512, omega=1.95, storage="f32", timing=True)
+    a = bgk.Lattice(512, 512, 512, omega=1.95, storage="f32", timing=True, three_step=False)
     a.init_hit(4.0, 0.02, 1234)
     a.step(10)
     ca = a.checksum()
@@ -316,13 +317,20 @@ def test_two_step_pass_512cubed_fp32_checksum(gpu):
 
 def test_two_step_pass_1040cubed(gpu):
     """1040^3 FP32 (84 GB of populations, tiles not dividing the lattice):
-    two-step passes == single steps."""
+    two-step passes (2 + 2) and a three-step pass (3 + 1) == single steps."""
     n = 1040
     say("two-step 1040^3 f32")
-    a = bgk.Lattice(n, n, n, omega=1.95, storage="f32", timing=True)
+    a = bgk.Lattice(n, n, n, omega=1.95, storage="f32", timing=True, three_step=False)
     a.init_hit(4.0, 0.02, 77)
     a.step(4)
     ca = a.checksum()
+    assert a.timing()["collide_launches"] == 2
+    a.close()
+    say("three-step 1040^3 f32")
+    a = bgk.Lattice(n, n, n, omega=1.95, storage="f32", timing=True)
+    a.init_hit(4.0, 0.02, 77)
+    a.step(4)
+    assert a.checksum() == ca
     assert a.timing()["collide_launches"] == 2
     a.close()
     b = bgk.Lattice(n, n, n, omega=1.95, storage="f32", two_step=False)
@@ -338,7 +346,7 @@ def test_two_step_pass_256cubed_checksum(gpu, kind):
     steps) and matches single steps bit for bit."""
     say("two-step 256^3", kind)
     u = bgk.hit_velocity(256, 256, 256, 4.0, 0.02, 5)
-    a = bgk.Lattice(256, 256, 256, omega=1.95, storage=kind, timing=True)
+    a = bgk.Lattice(256, 256, 256, omega=1.95, storage=kind, timing=True, three_step=False)
     b = bgk.Lattice(256, 256, 256, omega=1.95, storage=kind, two_step=False)
     a.init_equilibrium(u)
     b.init_equilibrium(u)

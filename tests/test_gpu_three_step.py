@@ -1,0 +1,117 @@
+"""GPU parity of the three-step pass (k_collide3): bgk_step's fusion of three
+time steps per pass over the populations must give the populations of
+single steps bit for bit -- every intermediate value is rounded to the
+storage type as a store would round it.  Sizes leave partial tiles in x and
+y, partial last z chunks, lattices narrower than a tile (wrap within one
+tile) and step counts that end with a two-step pass or a single step."""
+import sys
+
+import numpy as np
+import pytest
+
+import bgk3d_hit_amd as bgk
+
+pytestmark = pytest.mark.gpu
+
+KINDS3 = ["f32", "f16", "bf16"]  # the storages the three-step pass serves
+
+
+def say(*a):
+    print("[gpu-case]", *a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.mark.parametrize("kind", KINDS3)
+@pytest.mark.parametrize("shifted", [True, False])
+@pytest.mark.parametrize("shape", [(70, 9, 6), (64, 16, 130), (24, 20, 16), (130, 17, 65),
+                                   (67, 23, 4), (5, 3, 7), (129, 31, 9)])
+def test_three_step_pass_equals_single_steps(gpu, kind, shifted, shape):
+    say("three-step vs single", kind, shifted, shape)
+    u = bgk.hit_velocity(*shape, 2.0, 0.03, 13)
+    a = bgk.Lattice(*shape, omega=1.93, storage=kind, shifted=shifted, two_step="always",
+                    timing=True)
+    b = bgk.Lattice(*shape, omega=1.93, storage=kind, shifted=shifted, two_step=False)
+    a.init_equilibrium(u)
+    b.init_equilibrium(u)
+    done = 0
+    for n in (3, 1, 5, 6, 4):
+        a.step(n)
+        for _ in range(n):
+            b.col_MC()
+        done += n
+        assert np.array_equal(a.get_populations(), b.get_populations()), (n, done)
+    # 3 | 1 | 3+2 | 3+3 | 3+1: eight launches for 19 steps
+    assert a.timing()["collide_launches"] == 8
+    a.close()
+    b.close()
+
+
+@pytest.mark.parametrize("kind", KINDS3)
+def test_three_step_equals_two_step_passes(gpu, kind):
+    """Three-step passes against two-step passes (BGK_FLAG_NO_THREE_STEP) on
+    a lattice where both engage by themselves: 24 steps, checksums equal."""
+    say("three-step vs two-step 192x160x128", kind)
+    shape = (192, 160, 128)
+    u = bgk.hit_velocity(*shape, 4.0, 0.02, 21)
+    a = bgk.Lattice(*shape, omega=1.95, storage=kind, timing=True)
+    b = bgk.Lattice(*shape, omega=1.95, storage=kind, three_step=False, timing=True)
+    a.init_equilibrium(u)
+    b.init_equilibrium(u)
+    a.step(24)
+    b.step(24)
+    assert a.timing()["collide_launches"] == 8
+    assert b.timing()["collide_launches"] == 12
+    assert a.checksum() == b.checksum()
+
+
+@pytest.mark.parametrize("kind", ["f64", "f32mix"])
+def test_three_step_not_taken_for_fp64_compute(gpu, kind):
+    """FP64 storage or FP64 compute: two-step passes only (the three-step
+    pass's LDS is sized for 4-byte values)."""
+    g = bgk.Lattice(128, 128, 128, omega=1.9, storage=kind, timing=True)
+    g.step(6)
+    assert g.timing()["collide_launches"] == 3
+
+
+@pytest.mark.parametrize("kind", ["f32", "f16"])
+def test_three_step_512cubed_checksum(gpu, kind):
+    """The headline lattice: 512^3, 12 steps in four three-step passes ==
+    12 single steps, bit for bit."""
+    say("three-step 512^3", kind)
+    a = bgk.Lattice(512, 512, 512, omega=1.95, storage=kind, timing=True)
+    a.init_hit(4.0, 0.02, 1234)
+    a.step(12)
+    ca = a.checksum()
+    assert a.timing()["collide_launches"] == 4
+    a.close()
+    b = bgk.Lattice(512, 512, 512, omega=1.95, storage=kind, two_step=False)
+    b.init_hit(4.0, 0.02, 1234)
+    b.step(12)
+    assert b.checksum() == ca
+    b.close()
+
+
+def test_three_step_with_diagnostics_between(gpu):
+    """Diagnostics between fused passes read the state the single steps
+    leave (FP32, 256^3, diagnostics every 10 steps = 3+3+3+1)."""
+    say("three-step diagnostics 256^3")
+    nu = bgk.svisc_from_omega(1.95)
+    a = bgk.Lattice(256, 256, 256, omega=1.95, storage="f32")
+    b = bgk.Lattice(256, 256, 256, omega=1.95, storage="f32", two_step=False)
+    a.init_hit(4.0, 0.02, 8)
+    b.init_hit(4.0, 0.02, 8)
+    for _ in range(3):
+        a.step(10)
+        b.step(10)
+        da, sa = a.diagnostics(nu)
+        db, sb = b.diagnostics(nu)
+        assert np.array_equal(da, db) and np.array_equal(sa, sb)
+
+
+@pytest.mark.parametrize("n,launches", [(64, 20), (128, 7), (256, 7)])
+def test_three_step_engages_where_two_step_does(gpu, n, launches):
+    """20 steps: 64^3 takes single steps (too few workgroups to fill the
+    GPU); 128^3 and up take six three-step passes and one two-step pass."""
+    g = bgk.Lattice(n, n, n, omega=1.9, storage="f32", timing=True)
+    g.step(20)
+    assert g.timing()["collide_launches"] == launches
