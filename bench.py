@@ -8,18 +8,21 @@ The headline `value` is the FP32 run (storage and compute FP32); the FP64
 and reduced-storage (FP16 / BF16 shifted f_i - w_i, FP32 collide) runs of
 the same workload are reported under `variants`.
 
-The time step is the two-step pass (k_collide2: two time steps per launch,
-the first step's state kept in LDS), which bgk_step takes on lattices this
-size; the one-pass-per-step kernel (k_collide) is reported beside it.
+bgk_step advances lattices this size three time steps per launch
+(k_collide3: the first two steps' states kept in LDS; FP32/FP16/BF16
+storage) and takes a two-step pass (k_collide2) for what is left of each
+50-step block and for FP64; the two-step pass and the one-pass-per-step
+kernel (k_collide) are reported beside it.
 
-`roofline` is measured live: HIP events around every collide launch on the
-stream it runs on give the average launch duration.  The unit of work is the
+`roofline` is measured live for the dominant kernel (the launch depth with
+the most collide time): HIP events around every collide launch on the stream
+it runs on give the average launch duration.  The unit of work is the
 site-update, whose algorithmic bytes are 19 loads + 19 stores (38 * s); a
-launch processes 512^3 site-updates (k_collide, one time step) or 2 * 512^3
-(k_collide2, two time steps), so `achieved` = 38 * s * 512^3 * steps per
-launch / launch time.  k_collide2 moves each population through HBM once per
-two time steps, so its `achieved` can exceed the HBM peak: `frac` > 1 is the
-factor by which it beats the bandwidth bound of any one-step-per-launch
+launch processes 512^3 site-updates per time step it advances (one, two or
+three), so `achieved` = 38 * s * 512^3 * steps per launch / launch time.
+The fused passes move each population through HBM once per two or three
+time steps, so their `achieved` can exceed the HBM peak: `frac` > 1 is the
+factor by which they beat the bandwidth bound of any one-step-per-launch
 kernel.  `traffic` is the HBM bytes per launch measured by rocprofv3
 FETCH_SIZE / WRITE_SIZE runs of this script (profiles/collide_traffic.json,
 null when absent or for another lattice size); `hbm_actual_gbs` = traffic /
@@ -45,6 +48,7 @@ METRIC = "MLUPS + % HBM roofline, 512^3 D3Q19 (FP64/FP32/FP16-store); adjoint/fw
 HBM_PEAK_GBS = 8000.0
 BYTES = {"f64": 8, "f32": 4, "f32mix": 4, "f16": 2, "bf16": 2}
 COMPUTE = {"f64": "f64", "f32": "f32", "f32mix": "f64", "f16": "f32", "bf16": "f32"}
+KERNEL = {1: "k_collide", 2: "k_collide2", 3: "k_collide3"}  # by time steps per launch
 
 
 def log(*a):
@@ -52,13 +56,13 @@ def log(*a):
 
 
 def run_gpu(kind, n, steps, warmup, diag_every, omega=1.95, shape=None, layout="periodic",
-            k0=4.0, urms=0.02, seed=1234, two_step=True):
+            k0=4.0, urms=0.02, seed=1234, two_step=True, three_step=True):
     import bgk3d_hit_amd as bgk
 
     shape = shape or (n, n, n)
     nu = bgk.svisc_from_omega(omega)
     lat = bgk.Lattice(*shape, omega=omega, storage=kind, layout=layout, timing=True,
-                      two_step=two_step)
+                      two_step=two_step, three_step=three_step)
     try:
         lat.init_hit(k0, urms, seed)
         d0 = lat.diagno()
@@ -77,27 +81,27 @@ def run_gpu(kind, n, steps, warmup, diag_every, omega=1.95, shape=None, layout="
                 diag.append((done + warmup, *d5, *s2))
         lat.synchronize()
         wall = time.perf_counter() - t0
+        passes = lat.timing_passes(reset=False)
         t = lat.timing(reset=True)
     finally:
         lat.close()
     nsite = shape[0] * shape[1] * shape[2]
     launches = max(t["collide_launches"], 1)
-    # per step: one collide launch (periodic), bulk + 2 boundary planes (slab),
-    # or half a launch (two-step pass)
     per_step = launches / steps
-    avg_ms = t["collide_ms"] / launches
     step_collide_ms = t["collide_ms"] / steps
-    two = per_step < 0.75
+    # the dominant kernel: the launch depth (time steps per launch) with the
+    # most collide time in the timed region
+    spl = max(passes, key=lambda k: passes[k][0])
+    avg_ms = passes[spl][0] / max(passes[spl][1], 1)
     # algorithmic bytes of the site-updates one launch processes: 38 * s per
-    # site-update, nsite per time step, one or two time steps per launch
-    algo_bytes = 38 * BYTES[kind] * nsite * (2 if two else 1)
-    pass_ms = step_collide_ms * (2 if two else 1)
-    achieved = algo_bytes / (pass_ms * 1e-3) / 1e9
+    # site-update, nsite per time step, spl time steps per launch
+    algo_bytes = 38 * BYTES[kind] * nsite * spl
+    achieved = algo_bytes / (avg_ms * 1e-3) / 1e9
     return {
         "kind": kind,
-        "two_step": two,
-        "kernel": ("k_collide2" if two else "k_collide") + f"<{kind}>",
-        "steps_per_launch": 2 if two else 1,
+        "kernel": KERNEL[spl] + f"<{kind}>",
+        "steps_per_launch": spl,
+        "launches_by_depth": {str(k): v[1] for k, v in passes.items() if v[1]},
         "mlups": nsite * steps / wall / 1e6,
         "ms_per_step": wall * 1e3 / steps,
         "collide_avg_ms": avg_ms,
@@ -109,7 +113,6 @@ def run_gpu(kind, n, steps, warmup, diag_every, omega=1.95, shape=None, layout="
         "achieved_gbs": achieved,
         "frac": achieved / HBM_PEAK_GBS,
         "algo_bytes_per_launch": algo_bytes,
-        "pass_ms": pass_ms,
         "nsite": nsite,
         "ke0": d0[4],
         "diag_last": diag[-1] if diag else None,
@@ -122,7 +125,7 @@ def summary(r, ref=None):
            "kernel": r["kernel"], "steps_per_launch": r["steps_per_launch"],
            "achieved_gbs": round(r["achieved_gbs"], 1), "frac": round(r["frac"], 4),
            "storage_bytes": BYTES[r["kind"]], "compute": COMPUTE[r["kind"]],
-           "traffic": load_traffic(r["kind"], r["two_step"], r["nsite"])}
+           "traffic": load_traffic(r["kind"], r["steps_per_launch"], r["nsite"])}
     if out["traffic"]:
         out["hbm_actual_gbs"] = round(out["traffic"] / (r["collide_avg_ms"] * 1e-3) / 1e9, 1)
     if ref is not None:
@@ -270,7 +273,7 @@ def calibrate_copy(gib, reps=10):
                       "avg_ms": ms, "gbs": nbytes / (ms * 1e-3) / 1e9}), flush=True)
 
 
-def load_traffic(kind, two_step, nsite):
+def load_traffic(kind, steps_per_launch, nsite):
     """HBM bytes per launch of the collide kernel at 512^3 (rocprofv3 counters)."""
     if nsite != 512 ** 3:
         return None
@@ -278,7 +281,7 @@ def load_traffic(kind, two_step, nsite):
     try:
         with open(p) as f:
             d = json.load(f)
-        key = ("k_collide2/" if two_step else "k_collide/") + kind
+        key = KERNEL[steps_per_launch] + "/" + kind
         return d.get(key, {}).get("hbm_bytes_per_launch")
     except (OSError, ValueError):
         return None
@@ -302,7 +305,7 @@ def copy_ceiling(nsite):
 
 
 def roofline(r):
-    traffic = load_traffic(r["kind"], r["two_step"], r["nsite"])
+    traffic = load_traffic(r["kind"], r["steps_per_launch"], r["nsite"])
     out = {"bound": "hbm", "achieved": round(r["achieved_gbs"], 1),
            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(r["frac"], 4),
            "traffic": traffic, "kernel": r["kernel"],
@@ -354,22 +357,31 @@ def main():
     log(f"[bench] f32: {main_run['mlups']:.0f} MLUPS, {main_run['kernel']} "
         f"{main_run['collide_avg_ms']:.3f} ms/launch = {main_run['achieved_gbs']:.0f} GB/s")
     variants = {}
-    single_ref = None
+    refs = {}  # the f32 run of each mode, the reference of the other kinds' speedups
+    modes = (("", {}), ("_two_step_pass", {"three_step": False}),
+             ("_one_pass_per_step", {"two_step": False}))
     for kind in ["f32"] + [k for k in a.variants.split(",") if k and k != "f32"]:
-        for two in ((True, False) if main_run["two_step"] else (True,)):
-            if kind == "f32" and two:
+        depth = 3
+        for suffix, kw in modes:
+            name = kind + suffix
+            if kind == "f32" and not suffix:
+                refs[suffix] = main_run
+                depth = main_run["steps_per_launch"]
                 continue
-            name = kind if two else kind + "_one_pass_per_step"
+            if (suffix == "_two_step_pass" and depth < 3) or (suffix and depth < 2):
+                continue  # the same kernel as the default run
             log(f"[bench] {name} {a.size}^3")
             r = guarded(name, lambda: run_gpu(kind, a.size, a.steps, a.warmup, a.diag_every,
-                                              two_step=two))
+                                              **kw))
             if "error" in r:
                 variants[name] = r
                 continue
             log(f"[bench] {name}: {r['mlups']:.0f} MLUPS, {r['kernel']} {r['achieved_gbs']:.0f} GB/s")
             if kind == "f32":
-                single_ref = r
-            variants[name] = summary(r, main_run if two else single_ref)
+                refs[suffix] = r
+            if not suffix:
+                depth = r["steps_per_launch"]
+            variants[name] = summary(r, refs.get(suffix))
     configs = {}
     if not a.no_extra:
         log("[bench] slab 1024x1024x256")

@@ -72,6 +72,13 @@ static hipError_t harvest(bgk_ctx *c) {
     return hipSuccess;
 }
 
+static void timing_clear(Timing &T) {
+    for (int k = 0; k < TCLS_N; k++) {
+        T.ms[k] = 0;
+        T.launches[k] = 0;
+    }
+}
+
 hipError_t timing_begin(bgk_ctx *c, int cls, hipStream_t s, hipEvent_t *out) {
     *out = nullptr;
     if (!c->timing.on) return hipSuccess;
@@ -179,9 +186,9 @@ int step_periodic(bgk_ctx *c, int nsteps, bool count) {
     if (!images && three_step_ok(c->lat)) {
         for (; nsteps >= 3; nsteps -= 3) {
             hipEvent_t ev;
-            BGK_HIP(timing_begin(c, TCLS_COLLIDE, c->s0, &ev));
+            BGK_HIP(timing_begin(c, TCLS_PASS3, c->s0, &ev));
             BGK_HIP(launch_collide3(c->lat, c->cur, c->omega, c->s0));
-            BGK_HIP(timing_end(c, TCLS_COLLIDE, c->s0, ev));
+            BGK_HIP(timing_end(c, TCLS_PASS3, c->s0, ev));
             c->cur ^= 1;
         }
     }
@@ -194,10 +201,10 @@ int step_periodic(bgk_ctx *c, int nsteps, bool count) {
                 BGK_HIP(launch_pack_faces(c->lat, a, c->gcache[1 - a], c->s0));
                 BGK_HIP(timing_end(c, TCLS_HALO, c->s0, ev));
             }
-            BGK_HIP(timing_begin(c, TCLS_COLLIDE, c->s0, &ev));
+            BGK_HIP(timing_begin(c, TCLS_PASS2, c->s0, &ev));
             BGK_HIP(launch_collide2(c->lat, a, c->omega, images ? c->gcache[a] : nullptr,
                                     c->s0));
-            BGK_HIP(timing_end(c, TCLS_COLLIDE, c->s0, ev));
+            BGK_HIP(timing_end(c, TCLS_PASS2, c->s0, ev));
             c->cur ^= 1;
         }
     }
@@ -276,9 +283,9 @@ static int step_slab2(bgk_ctx *c, int npairs) {
         BGK_HIP(hipStreamWaitEvent(c->s1, c->ev0, 0));  // bulk(k-1) done
         BGK_HIP(hipStreamWaitEvent(c->s0, c->ev1, 0));  // boundary(k-1) done
         hipEvent_t ev;
-        BGK_HIP(timing_begin(c, TCLS_COLLIDE, c->s0, &ev));
+        BGK_HIP(timing_begin(c, TCLS_PASS2, c->s0, &ev));
         BGK_HIP(launch_collide2(L, a, c->omega, nullptr, c->s0));
-        BGK_HIP(timing_end(c, TCLS_COLLIDE, c->s0, ev));
+        BGK_HIP(timing_end(c, TCLS_PASS2, c->s0, ev));
         BGK_HIP(timing_begin(c, TCLS_COLLIDE, c->s1, &ev));
         BGK_HIP(launch_collide_planes(L, L.pop[a], L.zoff, c->slab_s1_pop, 1, lo3, c->omega,
                                       c->s1));
@@ -820,20 +827,30 @@ int bgk_timing(bgk_ctx *c, double t[8], int reset) {
     if (!t) return fail(BGK_EINVAL, "null output");
     BGK_HIP(harvest(c));
     Timing &T = c->timing;
-    t[0] = T.ms[TCLS_COLLIDE];
-    t[1] = (double)T.launches[TCLS_COLLIDE];
+    t[0] = T.ms[TCLS_COLLIDE] + T.ms[TCLS_PASS2] + T.ms[TCLS_PASS3];
+    t[1] = (double)(T.launches[TCLS_COLLIDE] + T.launches[TCLS_PASS2] + T.launches[TCLS_PASS3]);
     t[2] = T.ms[TCLS_REDUCE];
     t[3] = (double)T.launches[TCLS_REDUCE];
     t[4] = T.ms[TCLS_HALO];
     t[5] = (double)T.launches[TCLS_HALO];
     t[6] = T.ms[TCLS_STEP];
     t[7] = (double)T.launches[TCLS_STEP];
-    if (reset) {
-        for (int k = 0; k < 4; k++) {
-            T.ms[k] = 0;
-            T.launches[k] = 0;
-        }
+    if (reset) timing_clear(T);
+    return BGK_OK;
+}
+
+int bgk_timing_passes(bgk_ctx *c, double t[6], int reset) {
+    int r = check_ctx(c);
+    if (r) return r;
+    if (!t) return fail(BGK_EINVAL, "null output");
+    BGK_HIP(harvest(c));
+    Timing &T = c->timing;
+    const int cls[3] = {TCLS_COLLIDE, TCLS_PASS2, TCLS_PASS3};
+    for (int k = 0; k < 3; k++) {
+        t[2 * k] = T.ms[cls[k]];
+        t[2 * k + 1] = (double)T.launches[cls[k]];
     }
+    if (reset) timing_clear(T);
     return BGK_OK;
 }
 

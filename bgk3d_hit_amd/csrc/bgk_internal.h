@@ -77,8 +77,8 @@ struct Timing {
         hipEvent_t a, b;
     };
     std::vector<Rec> recs;
-    double ms[4] = {0, 0, 0, 0};
-    long long launches[4] = {0, 0, 0, 0};
+    double ms[6] = {0, 0, 0, 0, 0, 0};
+    long long launches[6] = {0, 0, 0, 0, 0, 0};
 };
 
 }  // namespace bgk
@@ -139,7 +139,10 @@ int hip_fail(hipError_t e, const char *what);
 size_t storage_bytes(int storage);
 
 // timing helpers (no-ops unless BGK_FLAG_TIMING)
-enum { TCLS_COLLIDE = 0, TCLS_REDUCE = 1, TCLS_HALO = 2, TCLS_STEP = 3 };
+// TCLS_PASS2 / TCLS_PASS3: collide launches of two / three time steps
+// (counted with the collide launches in bgk_timing)
+enum { TCLS_COLLIDE = 0, TCLS_REDUCE = 1, TCLS_HALO = 2, TCLS_STEP = 3, TCLS_PASS2 = 4,
+       TCLS_PASS3 = 5, TCLS_N = 6 };
 hipError_t timing_begin(bgk_ctx *c, int cls, hipStream_t s, hipEvent_t *out);
 hipError_t timing_end(bgk_ctx *c, int cls, hipStream_t s, hipEvent_t begin);
 

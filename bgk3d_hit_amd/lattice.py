@@ -198,6 +198,13 @@ class Lattice:
                 "reduce_launches": int(t[3]), "halo_ms": t[4], "halo_launches": int(t[5]),
                 "step_ms": t[6], "step_calls": int(t[7])}
 
+    def timing_passes(self, reset=True):
+        """Collide launches split by the time steps one launch advances:
+        {steps_per_launch: (ms, launches)} for 1, 2 and 3."""
+        t = np.zeros(6)
+        L.check(self._lib.bgk_timing_passes(self._h, _dptr(t), 1 if reset else 0))
+        return {k + 1: (t[2 * k], int(t[2 * k + 1])) for k in range(3)}
+
     # -- adjoint
     def adjoint_ke(self, u0, nsteps, ckpt_every=25, levels=1):
         """(J, dJ/du0, stats) for J = sum |u|^2/2 after nsteps from equilibrium(u0)."""

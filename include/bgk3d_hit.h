@@ -194,6 +194,14 @@ int bgk_synchronize(bgk_ctx *ctx);
  * timers (SRC/mod_tools.F90:11-54, SRC/finalize.F90:44-97).                 */
 int bgk_timing(bgk_ctx *ctx, double t[8], int reset);
 
+/* With BGK_FLAG_TIMING: the collide launches of bgk_timing's t[0..1] split by
+ * the time steps one launch advances: t[0], t[1] = ms and launches of one
+ * step per launch (k_collide, also the slab's boundary planes), t[2], t[3]
+ * of two-step passes (k_collide2), t[4], t[5] of three-step passes
+ * (k_collide3).  reset != 0 clears every counter (as bgk_timing's does).
+ * No counterpart in the reference (its timers are per section).             */
+int bgk_timing_passes(bgk_ctx *ctx, double t[6], int reset);
+
 /* Discrete adjoint (not in the reference).  J = sum_x |u(x)|^2 / 2 of the
  * state reached after nsteps steps from the equilibrium of u0 (rho = 1);
  * grad = dJ/du0 (3 planes).  Checkpoints every ckpt_every steps with
