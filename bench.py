@@ -9,10 +9,9 @@ and reduced-storage (FP16 / BF16 shifted f_i - w_i, FP32 collide) runs of
 the same workload are reported under `variants`.
 
 bgk_step advances lattices this size three time steps per launch
-(k_collide3: the first two steps' states kept in LDS; FP32/FP16/BF16
-storage) and takes a two-step pass (k_collide2) for what is left of each
-50-step block and for FP64; the two-step pass and the one-pass-per-step
-kernel (k_collide) are reported beside it.
+(k_collide3: the first two steps' states kept in LDS) and takes a two-step
+pass (k_collide2) for what is left of each 50-step block; the two-step pass
+alone and the one-pass-per-step kernel (k_collide) are reported beside it.
 
 `roofline` is measured live for the dominant kernel (the launch depth with
 the most collide time): HIP events around every collide launch on the stream

@@ -681,9 +681,14 @@ __global__ __launch_bounds__(collide2_threads(TX, TY), MINB) void k_collide2(
 // (TX+4)(TY+4)/(TX TY) + (TX+2)(TY+2)/(TX TY) + 1 (64 x 10: 3.73, i.e. 1.24
 // per step, against 1.21 per step of the two-step pass's 128 x 5 tiles),
 // while the populations cross HBM once per three steps instead of two.
-template <typename S>
+// FP64 compute: 32 x 10 own tiles (36 x 14 grown, 146 KB of LDS in FP64),
+// 512 threads, so that a thread may hold 256 VGPRs (collisions per step:
+// 1.28).  Measured at 512^3 FP64, ms per pass: 32 x 10 11.73, 32 x 8 12.58,
+// 32 x 6 13.70, 64 x 3 14.01 (64 x 4 would spill).
+template <typename S, typename C>
 struct Tile3 {
-    static constexpr int X = 64, Y = 10, ZC = 64;
+    static constexpr bool f64 = sizeof(C) == 8;
+    static constexpr int X = f64 ? 32 : 64, Y = 10, ZC = 64;
 };
 
 // threads: one per site of the TX-wide x range of the TY+4 grown rows, plus

@@ -5,7 +5,7 @@
 //                      in the index math (or slab ghost planes in z)
 //   k_collide2         two time steps per pass (launchers here, kernel in
 //                      bgk_collide.h); on the slab for the bulk planes
-//   k_collide3         three time steps per pass (periodic, 4/2-byte storage)
+//   k_collide3         three time steps per pass (periodic layout)
 //   k_diagno           at-site moment sums (SRC/diagno.F90:73-115)
 //   k_dissipation      pulled Pi^neq / kinetic energy sums
 //                      (SRC/dissipation.F90:70-253)
@@ -76,10 +76,10 @@ static hipError_t collide2_t(const Lattice &L, int src, double omega, void *face
 }
 
 // three steps from copy `src` into copy 1-src (periodic lattice, 32-bit
-// offsets, 4- or 2-byte storage computed in FP32)
+// offsets)
 template <typename S, typename C, bool SH>
 static hipError_t collide3_sh(const Lattice &L, int src, double omega, hipStream_t s) {
-    using T = Tile3<S>;
+    using T = Tile3<S, C>;
     const int zc = three_step_zc(L, T::X, T::Y, T::ZC);
     if (zc <= 0) return hipErrorInvalidValue;  // caller checks three_step_ok first
     int nblocks;
@@ -91,12 +91,8 @@ static hipError_t collide3_sh(const Lattice &L, int src, double omega, hipStream
 }
 template <typename S, typename C>
 static hipError_t collide3_t(const Lattice &L, int src, double omega, hipStream_t s) {
-    if constexpr (sizeof(S) == 8 || sizeof(C) == 8) {
-        return hipErrorInvalidValue;  // not offered (three_step_ok)
-    } else {
-        return L.shifted ? collide3_sh<S, C, true>(L, src, omega, s)
-                         : collide3_sh<S, C, false>(L, src, omega, s);
-    }
+    return L.shifted ? collide3_sh<S, C, true>(L, src, omega, s)
+                     : collide3_sh<S, C, false>(L, src, omega, s);
 }
 
 // --------------------------------------------------------------- reductions
@@ -717,18 +713,21 @@ bool two_step_ok(const Lattice &L) {
     default: return false;
     }
 }
+template <typename S, typename C>
+static bool three_step_ok_t(const Lattice &L) {
+    using T = Tile3<S, C>;
+    return offsets_fit_32(L, sizeof(S)) && three_step_zc(L, T::X, T::Y, T::ZC) > 0;
+}
 bool three_step_ok(const Lattice &L) {
     if (!L.two_step || !L.three_step || L.layout != BGK_LAYOUT_PERIODIC) return false;
-    size_t es;
     switch (L.storage) {
-    case BGK_STORE_F32: es = 4; break;
-    case BGK_STORE_F16:
-    case BGK_STORE_BF16: es = 2; break;
+    case BGK_STORE_F64: return three_step_ok_t<double, double>(L);
+    case BGK_STORE_F32: return three_step_ok_t<float, float>(L);
+    case BGK_STORE_F32_MIXED: return three_step_ok_t<float, double>(L);
+    case BGK_STORE_F16: return three_step_ok_t<half_t, float>(L);
+    case BGK_STORE_BF16: return three_step_ok_t<bf16_t, float>(L);
     default: return false;
     }
-    if (!offsets_fit_32(L, es)) return false;
-    return es == 4 ? three_step_zc(L, Tile3<float>::X, Tile3<float>::Y, Tile3<float>::ZC) > 0
-                   : three_step_zc(L, Tile3<half_t>::X, Tile3<half_t>::Y, Tile3<half_t>::ZC) > 0;
 }
 hipError_t launch_collide3(const Lattice &L, int src, double omega, hipStream_t s) {
     BGK_DISPATCH(L.storage, collide3_t, L, src, omega, s);

@@ -53,8 +53,7 @@ class Lattice:
     two_step: True fuses two time steps per pass in step() where the lattice
     is large enough for it to pay, False takes one pass per step, "always"
     fuses at any size (testing).  three_step: with two_step, also fuse three
-    time steps per pass (periodic layout, FP32/FP16/BF16 storage) where
-    two_step would fuse two."""
+    time steps per pass (periodic layout) where two_step would fuse two."""
 
     def __init__(self, lx, ly, lz, omega, storage="f64", layout="periodic", shifted=True,
                  refcompat_halos=False, timing=False, two_step=True, wide_offsets=False,

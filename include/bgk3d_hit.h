@@ -140,7 +140,7 @@ int bgk_col_MC_masked(bgk_ctx *ctx, int opt);
 /* nsteps of the time loop body boundaries + col_MC (SRC/bgk3d_HIT.F90:127-146
  * without diagnostic).  Where the lattice is large enough (unless
  * BGK_FLAG_SINGLE_STEP): three steps per pass over the populations (periodic
- * layout, FP32/FP16/BF16 storage, no halo images; unless
+ * layout, no halo images; unless
  * BGK_FLAG_NO_THREE_STEP), then two steps per pass, then single steps;
  * bit-identical to nsteps calls of bgk_col_MC, stale halo images included.  Slab layout: the interior collide overlaps the halo
  * pack/exchange/unpack and boundary collide on a second HIP stream (the

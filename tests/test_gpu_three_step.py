@@ -13,7 +13,7 @@ import bgk3d_hit_amd as bgk
 
 pytestmark = pytest.mark.gpu
 
-KINDS3 = ["f32", "f16", "bf16"]  # the storages the three-step pass serves
+KINDS3 = ["f64", "f32", "f32mix", "f16", "bf16"]
 
 
 def say(*a):
@@ -64,16 +64,7 @@ def test_three_step_equals_two_step_passes(gpu, kind):
     assert a.checksum() == b.checksum()
 
 
-@pytest.mark.parametrize("kind", ["f64", "f32mix"])
-def test_three_step_not_taken_for_fp64_compute(gpu, kind):
-    """FP64 storage or FP64 compute: two-step passes only (the three-step
-    pass's LDS is sized for 4-byte values)."""
-    g = bgk.Lattice(128, 128, 128, omega=1.9, storage=kind, timing=True)
-    g.step(6)
-    assert g.timing()["collide_launches"] == 3
-
-
-@pytest.mark.parametrize("kind", ["f32", "f16"])
+@pytest.mark.parametrize("kind", ["f64", "f32", "f16"])
 def test_three_step_512cubed_checksum(gpu, kind):
     """The headline lattice: 512^3, 12 steps in four three-step passes ==
     12 single steps, bit for bit."""
