@@ -184,7 +184,11 @@ int step_periodic(bgk_ctx *c, int nsteps, bool count) {
     const bool images = (c->flags & BGK_FLAG_REFCOMPAT_HALOS) != 0;
     const int n0 = nsteps;
     if (!images && three_step_ok(c->lat)) {
-        for (; nsteps >= 3; nsteps -= 3) {
+        // a remainder of one step: two two-step passes rather than a
+        // three-step pass and a single step (512^3 FP32: 8.0 against 8.9 ms)
+        int n3 = nsteps / 3;
+        if (nsteps % 3 == 1 && n3 > 0 && two_step_ok(c->lat)) n3--;
+        for (; n3 > 0; n3--, nsteps -= 3) {
             hipEvent_t ev;
             BGK_HIP(timing_begin(c, TCLS_PASS3, c->s0, &ev));
             BGK_HIP(launch_collide3(c->lat, c->cur, c->omega, c->s0));

@@ -53,7 +53,7 @@ end module bgk_host
 program bgk_HIT
   use bgk_host
   implicit none
-  integer :: itime, itstart
+  integer :: itime, itstart, nrun
   integer(8) :: c0, c1, e0, e1
   character(len=256) :: deck
   logical, external :: step_event
@@ -76,11 +76,15 @@ program bgk_HIT
   itime = itstart
   do while (itime <= itfin)                      ! SRC/bgk3d_HIT.F90:127-146
      call boundaries(itime)
-     if (itime < itfin .and. .not. step_event(itime)) then
-        ! nothing is due after step itime: steps itime and itime+1 in one
-        ! call, which the library runs as one two-step pass where it can
-        call col_MC2(itime)
-        itime = itime + 1
+     ! the steps up to the next one followed by output (or itfin) in one
+     ! call, which the library runs as three- and two-step passes where it can
+     nrun = 1
+     do while (itime + nrun - 1 < itfin .and. .not. step_event(itime + nrun - 1))
+        nrun = nrun + 1
+     end do
+     if (nrun > 1) then
+        call col_MCn(itime, nrun)
+        itime = itime + nrun - 1
      else
         call col_MC(itime)
      end if
@@ -267,21 +271,21 @@ subroutine col_MC(itime)
   if (itime < 0) write(6, *) itime
 end subroutine col_MC
 
-! Steps itime and itime+1 (boundaries of itime+1 included: nothing to exchange
-! on the periodic lattice; with refcompat halos the library keeps the halo
-! images of both steps itself)
-subroutine col_MC2(itime)
+! Steps itime .. itime+nrun-1 (their boundaries included: nothing to
+! exchange on the periodic lattice; with refcompat halos the library keeps
+! the halo images of every step itself)
+subroutine col_MCn(itime, nrun)
   use bgk_host
   implicit none
-  integer, intent(in) :: itime
+  integer, intent(in) :: itime, nrun
   integer(8) :: c0, c1
   call system_clock(c0)
-  call bgk_check(bgk_step(ctx, 2_c_int), 'bgk_step')
+  call bgk_check(bgk_step(ctx, int(nrun, c_int)), 'bgk_step')
   call bgk_check(bgk_synchronize(ctx), 'bgk_synchronize')
   call system_clock(c1)
   time_coll = time_coll + real(c1 - c0, dp) / real(count_rate, dp)
   if (itime < 0) write(6, *) itime
-end subroutine col_MC2
+end subroutine col_MCn
 
 ! whether step itime is followed by output (diagnostics, signal, restart)
 logical function step_event(itime)

@@ -317,27 +317,20 @@ def test_two_step_pass_512cubed_fp32_checksum(gpu):
 
 def test_two_step_pass_1040cubed(gpu):
     """1040^3 FP32 (84 GB of populations, tiles not dividing the lattice):
-    two-step passes (2 + 2) and a three-step pass (3 + 1) == single steps."""
+    three-step passes (3 + 3), two-step passes (2 + 2 + 2) and single steps
+    give the same populations."""
     n = 1040
-    say("two-step 1040^3 f32")
-    a = bgk.Lattice(n, n, n, omega=1.95, storage="f32", timing=True, three_step=False)
-    a.init_hit(4.0, 0.02, 77)
-    a.step(4)
-    ca = a.checksum()
-    assert a.timing()["collide_launches"] == 2
-    a.close()
-    say("three-step 1040^3 f32")
-    a = bgk.Lattice(n, n, n, omega=1.95, storage="f32", timing=True)
-    a.init_hit(4.0, 0.02, 77)
-    a.step(4)
-    assert a.checksum() == ca
-    assert a.timing()["collide_launches"] == 2
-    a.close()
-    b = bgk.Lattice(n, n, n, omega=1.95, storage="f32", two_step=False)
-    b.init_hit(4.0, 0.02, 77)
-    b.step(4)
-    assert b.checksum() == ca
-    b.close()
+    sums = []
+    for name, kw, launches in (("three-step", {}, 2), ("two-step", {"three_step": False}, 3),
+                               ("single", {"two_step": False}, 6)):
+        say(name, "1040^3 f32")
+        a = bgk.Lattice(n, n, n, omega=1.95, storage="f32", timing=True, **kw)
+        a.init_hit(4.0, 0.02, 77)
+        a.step(6)
+        sums.append(a.checksum())
+        assert a.timing()["collide_launches"] == launches, name
+        a.close()
+    assert sums[0] == sums[1] == sums[2]
 
 
 @pytest.mark.parametrize("kind", ["f32", "f16", "f64"])

@@ -40,8 +40,10 @@ def test_three_step_pass_equals_single_steps(gpu, kind, shifted, shape):
             b.col_MC()
         done += n
         assert np.array_equal(a.get_populations(), b.get_populations()), (n, done)
-    # 3 | 1 | 3+2 | 3+3 | 3+1: eight launches for 19 steps
-    assert a.timing()["collide_launches"] == 8
+    # 3 | 1 | 3+2 | 3+3 | 2+2 (a remainder of one after three-step passes
+    # goes to two two-step passes): eight launches for 19 steps
+    t = a.timing_passes()
+    assert (t[1][1], t[2][1], t[3][1]) == (1, 3, 4)
     a.close()
     b.close()
 
@@ -84,7 +86,7 @@ def test_three_step_512cubed_checksum(gpu, kind):
 
 def test_three_step_with_diagnostics_between(gpu):
     """Diagnostics between fused passes read the state the single steps
-    leave (FP32, 256^3, diagnostics every 10 steps = 3+3+3+1)."""
+    leave (FP32, 256^3, diagnostics every 10 steps = 3+3+2+2)."""
     say("three-step diagnostics 256^3")
     nu = bgk.svisc_from_omega(1.95)
     a = bgk.Lattice(256, 256, 256, omega=1.95, storage="f32")
@@ -106,3 +108,15 @@ def test_three_step_engages_where_two_step_does(gpu, n, launches):
     g = bgk.Lattice(n, n, n, omega=1.9, storage="f32", timing=True)
     g.step(20)
     assert g.timing()["collide_launches"] == launches
+
+
+@pytest.mark.parametrize("n,depths", [(1, (1, 0, 0)), (2, (0, 1, 0)), (4, (0, 2, 0)),
+                                      (7, (0, 2, 1)), (25, (0, 2, 7)), (50, (0, 1, 16))])
+def test_step_split_into_passes(gpu, n, depths):
+    """How bgk_step splits n steps: three-step passes, then a two-step pass
+    for a remainder of two, two for a remainder of one (cheaper than a
+    three-step pass and a single step), single steps only below two."""
+    g = bgk.Lattice(128, 128, 128, omega=1.9, storage="f32", timing=True)
+    g.step(n)
+    t = g.timing_passes()
+    assert (t[1][1], t[2][1], t[3][1]) == depths
