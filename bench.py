@@ -144,21 +144,29 @@ def slab_config(steps, warmup):
     out = {"workload": "one rank's slab 1024x1024x256 FP32, +-z ghost planes, pack / "
                        "self-exchange / unpack of the 5 crossing populations per face, "
                        "bulk || boundary on two HIP streams, k0=8 u_rms=0.03 omega=1.97"}
-    for two in (True, False):
+    paths = {
+        "three_step_pass": ({}, "three-step pass on planes 3..252 (stream 0) || halo exchange, "
+                                "boundary planes stepped three times through two plane buffers "
+                                "(stream 1); 16 passes and a two-step pass per 50 steps"),
+        "two_step_pass": ({"three_step": False},
+                          "two-step pass on planes 2..253 (stream 0) || halo exchange, "
+                          "boundary planes stepped twice through a plane buffer (stream 1)"),
+        "one_pass_per_step": ({"two_step": False},
+                              "one-step kernel on planes 1..254 (stream 0) || halo exchange "
+                              "and boundary planes (stream 1)"),
+    }
+    for name, (kw, path) in paths.items():
         r = run_gpu("f32", 0, steps, warmup, 50, omega=1.97, shape=(1024, 1024, 256),
-                    layout="slab", k0=8.0, urms=0.03, seed=5678, two_step=two)
+                    layout="slab", k0=8.0, urms=0.03, seed=5678, **kw)
         nsite = 1024 * 1024 * 256
         # two streams overlap, so the collide time summed over both is not a
         # kernel rate: report the whole step's one-step-equivalent rate
-        out["two_step_pass" if two else "one_pass_per_step"] = {
+        out[name] = {
             "mlups": round(r["mlups"], 1), "ms_per_step": round(r["ms_per_step"], 4),
-            "path": ("two-step pass on planes 2..253 (stream 0) || halo exchange, "
-                     "boundary planes stepped twice through a plane buffer (stream 1)")
-            if two else ("one-step kernel on planes 1..254 (stream 0) || halo exchange "
-                         "and boundary planes (stream 1)"),
+            "path": path,
             "one_step_equivalent_gbs": round(38 * 4 * nsite / (r["ms_per_step"] * 1e-3) / 1e9, 1),
             "halo_ms_per_step": round(r["halo_ms_per_step"], 4)}
-        log(f"[bench] slab {'two-step' if two else 'one pass per step'}: {r['mlups']:.0f} MLUPS")
+        log(f"[bench] slab {name}: {r['mlups']:.0f} MLUPS")
     return out
 
 

@@ -261,6 +261,13 @@ static int step_slab(bgk_ctx *c, int nsteps) {
     return BGK_OK;
 }
 
+// Ghost planes as single steps leave them (what the reference-compatible
+// dissipation reads, SRC/dissipation.F90:76-94): after step t the current
+// copy's ghosts hold the images of S_{t-2}, the other copy's those of
+// S_{t-1}.  A fused pass from S_T to S_{T+n} has both messages at hand (the
+// exchanges of its first-step and (n-1)-th-step planes, or of S_T itself)
+// and unpacks them into the two copies' ghosts as well.
+//
 // Two steps per pass on the slab.  Stream s0: the two-step pass over the
 // bulk planes 2..nz-3 (it reads planes 0..nz-1 of the source copy only).
 // Stream s1: the halo exchange of the source copy (the single step's
@@ -284,6 +291,8 @@ static int step_slab2(bgk_ctx *c, int npairs) {
         const int a = c->cur, b = 1 - c->cur;
         int r = halo_exchange(c, c->s1);  // does not depend on s0's previous pass
         if (r) return r;
+        // the destination's ghosts as single steps leave them: S_T's images
+        BGK_HIP(launch_unpack_planes(L, L.pop[b], 0, nz + 1, c->halo_send, c->s1));
         BGK_HIP(hipStreamWaitEvent(c->s1, c->ev0, 0));  // bulk(k-1) done
         BGK_HIP(hipStreamWaitEvent(c->s0, c->ev1, 0));  // boundary(k-1) done
         hipEvent_t ev;
@@ -299,6 +308,8 @@ static int step_slab2(bgk_ctx *c, int npairs) {
         BGK_HIP(timing_begin(c, TCLS_HALO, c->s1, &ev));
         BGK_HIP(launch_pack_planes(L, c->slab_s1_pop, 6, 1, c->halo_send, c->s1));
         BGK_HIP(launch_unpack_planes(L, c->slab_s1_pop, 0, 7, c->halo_send, c->s1));
+        // and the source's (read above) as single steps leave them: S_{T+1}'s
+        BGK_HIP(launch_unpack_planes(L, L.pop[a], 0, nz + 1, c->halo_send, c->s1));
         BGK_HIP(timing_end(c, TCLS_HALO, c->s1, ev));
         BGK_HIP(timing_begin(c, TCLS_COLLIDE, c->s1, &ev));
         BGK_HIP(launch_collide_planes(L, c->slab_s1_pop, 1, L.pop[b], L.zoff, lo2, c->omega,
@@ -310,6 +321,77 @@ static int step_slab2(bgk_ctx *c, int npairs) {
         BGK_HIP(hipEventRecord(c->ev1, c->s1));
         c->cur ^= 1;
         c->steps += 2;
+    }
+    // join: later work on s0 sees the boundary planes
+    BGK_HIP(hipStreamWaitEvent(c->s0, c->ev1, 0));
+    return BGK_OK;
+}
+
+// Three steps per pass on the slab, the two-step structure one level deeper.
+// Stream s0: the three-step pass over the bulk planes 3..nz-4 (it reads
+// planes 0..nz-1 of the source copy only).  Stream s1: the halo exchange of
+// the source copy; the first step of planes 0..4 and nz-5..nz-1 into plane
+// buffer B1 (slab_s1_pop: interior plane z at buffer plane z+1, resp.
+// z+11-nz), the exchange of B1's ghost planes 0 and 11; the second step of
+// planes 0..3 and nz-4..nz-1 into B2 (slab_s2_pop: z+1, resp. z+9-nz), the
+// exchange of B2's ghost planes 0 and 9; the third step of planes 0..2 and
+// nz-3..nz-1 into the destination copy.  One exchange of the 5 crossing
+// populations per face and step, as single steps.  Cross-stream
+// dependencies per pass k as in step_slab2 (the boundary path reads planes
+// 0..5 and nz-6..nz-1, the bulk writes planes 3..nz-4).
+static int step_slab3(bgk_ctx *c, int ntriples) {
+    const Lattice &L = c->lat;
+    const int nz = L.nz;
+    auto zbox = [&](int z0, int z1) { return Box{0, L.nx, 0, L.ny, z0, z1}; };
+    void *const *B1 = c->slab_s1_pop;
+    void *const *B2 = c->slab_s2_pop;
+    BGK_HIP(hipEventRecord(c->ev0, c->s0));
+    BGK_HIP(hipStreamWaitEvent(c->s1, c->ev0, 0));
+    BGK_HIP(hipEventRecord(c->ev1, c->s1));
+    for (int k = 0; k < ntriples; k++) {
+        const int a = c->cur, b = 1 - c->cur;
+        int r = halo_exchange(c, c->s1);  // does not depend on s0's previous pass
+        if (r) return r;
+        BGK_HIP(hipStreamWaitEvent(c->s1, c->ev0, 0));  // bulk(k-1) done
+        BGK_HIP(hipStreamWaitEvent(c->s0, c->ev1, 0));  // boundary(k-1) done
+        hipEvent_t ev;
+        BGK_HIP(timing_begin(c, TCLS_PASS3, c->s0, &ev));
+        BGK_HIP(launch_collide3(L, a, c->omega, c->s0));
+        BGK_HIP(timing_end(c, TCLS_PASS3, c->s0, ev));
+        // first step
+        BGK_HIP(timing_begin(c, TCLS_COLLIDE, c->s1, &ev));
+        BGK_HIP(launch_collide_planes(L, L.pop[a], L.zoff, B1, 1, zbox(0, 5), c->omega, c->s1));
+        BGK_HIP(launch_collide_planes(L, L.pop[a], L.zoff, B1, 11 - nz, zbox(nz - 5, nz),
+                                      c->omega, c->s1));
+        BGK_HIP(timing_end(c, TCLS_COLLIDE, c->s1, ev));
+        BGK_HIP(timing_begin(c, TCLS_HALO, c->s1, &ev));
+        BGK_HIP(launch_pack_planes(L, B1, 10, 1, c->halo_send, c->s1));
+        BGK_HIP(launch_unpack_planes(L, B1, 0, 11, c->halo_send, c->s1));
+        // the destination's ghosts as single steps leave them: S_{T+1}'s images
+        BGK_HIP(launch_unpack_planes(L, L.pop[b], 0, nz + 1, c->halo_send, c->s1));
+        BGK_HIP(timing_end(c, TCLS_HALO, c->s1, ev));
+        // second step
+        BGK_HIP(timing_begin(c, TCLS_COLLIDE, c->s1, &ev));
+        BGK_HIP(launch_collide_planes(L, B1, 1, B2, 1, zbox(0, 4), c->omega, c->s1));
+        BGK_HIP(launch_collide_planes(L, B1, 11 - nz, B2, 9 - nz, zbox(nz - 4, nz), c->omega,
+                                      c->s1));
+        BGK_HIP(timing_end(c, TCLS_COLLIDE, c->s1, ev));
+        BGK_HIP(timing_begin(c, TCLS_HALO, c->s1, &ev));
+        BGK_HIP(launch_pack_planes(L, B2, 8, 1, c->halo_send, c->s1));
+        BGK_HIP(launch_unpack_planes(L, B2, 0, 9, c->halo_send, c->s1));
+        // and the source's (read above) as single steps leave them: S_{T+2}'s
+        BGK_HIP(launch_unpack_planes(L, L.pop[a], 0, nz + 1, c->halo_send, c->s1));
+        BGK_HIP(timing_end(c, TCLS_HALO, c->s1, ev));
+        // third step into the destination copy
+        BGK_HIP(timing_begin(c, TCLS_COLLIDE, c->s1, &ev));
+        BGK_HIP(launch_collide_planes(L, B2, 1, L.pop[b], L.zoff, zbox(0, 3), c->omega, c->s1));
+        BGK_HIP(launch_collide_planes(L, B2, 9 - nz, L.pop[b], L.zoff, zbox(nz - 3, nz),
+                                      c->omega, c->s1));
+        BGK_HIP(timing_end(c, TCLS_COLLIDE, c->s1, ev));
+        BGK_HIP(hipEventRecord(c->ev0, c->s0));
+        BGK_HIP(hipEventRecord(c->ev1, c->s1));
+        c->cur ^= 1;
+        c->steps += 3;
     }
     // join: later work on s0 sees the boundary planes
     BGK_HIP(hipStreamWaitEvent(c->s0, c->ev1, 0));
@@ -444,15 +526,18 @@ int bgk_create(int lx, int ly, int lz, int storage, int layout, int flags, doubl
         if ((e = hipMalloc(&c->halo_send, c->halo_bytes)) != hipSuccess)
             return bail(hip_fail(e, "hipMalloc(halo)"));
         c->bytes += c->halo_bytes;
-        if (L.two_step) {  // first-step plane buffer of the two-step boundary path:
-                           // 8 planes in the lattice's layout (same pitches)
-            const size_t pb8 = ((size_t)8 * L.zstride * es + 255) & ~(size_t)255;
-            if ((e = hipMalloc(&c->slab_s1, pb8)) != hipSuccess)
+        if (L.two_step) {  // plane buffers of the fused boundary paths: 22
+                           // planes in the lattice's layout (same pitches)
+            const size_t pbytes = (size_t)L.zstride * es;
+            const size_t pb = (22 * pbytes + 255) & ~(size_t)255;
+            if ((e = hipMalloc(&c->slab_s1, pb)) != hipSuccess)
                 return bail(hip_fail(e, "hipMalloc(slab plane buffer)"));
-            c->bytes += pb8;
-            for (int q = 0; q < 19; q++)
+            c->bytes += pb;
+            for (int q = 0; q < 19; q++) {
                 c->slab_s1_pop[q] = (char *)c->slab_s1 + (size_t)q * lx * es;
-            if ((e = hipMemsetAsync(c->slab_s1, 0, pb8, c->s0)) != hipSuccess)
+                c->slab_s2_pop[q] = (char *)c->slab_s1 + 12 * pbytes + (size_t)q * lx * es;
+            }
+            if ((e = hipMemsetAsync(c->slab_s1, 0, pb, c->s0)) != hipSuccess)
                 return bail(hip_fail(e, "hipMemsetAsync(slab plane buffer)"));
         }
     }
@@ -661,7 +746,13 @@ int bgk_step(bgk_ctx *c, int nsteps) {
     BGK_HIP(timing_begin(c, TCLS_STEP, c->s0, &ev));
     if (c->lat.layout == BGK_LAYOUT_SLAB_Z) {
         r = BGK_OK;
-        if (two_step_ok(c->lat)) {
+        if (three_step_ok(c->lat)) {  // remainders as on the periodic lattice
+            int n3 = nsteps / 3;
+            if (nsteps % 3 == 1 && n3 > 0 && two_step_ok(c->lat)) n3--;
+            r = step_slab3(c, n3);
+            nsteps -= 3 * n3;
+        }
+        if (!r && two_step_ok(c->lat)) {
             r = step_slab2(c, nsteps / 2);
             nsteps %= 2;
         }

@@ -673,8 +673,10 @@ __global__ __launch_bounds__(collide2_threads(TX, TY), MINB) void k_collide2(
 // group/slot layout of the two-step pass (20 planes each); the z-only ones
 // stay in the column's registers.  Every intermediate value is rounded to the
 // storage type exactly as a store and reload would round it, so a pass equals
-// three single steps bit for bit.  Periodic lattice, 32-bit offsets through
-// one buffer resource per z plane (populations g.pstride bytes apart).
+// three single steps bit for bit.  Periodic lattice (wrap in z), or a slab's
+// bulk, whose pass reads only stored planes zlo-3 .. zhi+2, all inside the
+// slab; 32-bit offsets through one buffer resource per z plane (populations
+// g.pstride bytes apart).
 //
 // Cost against the two-step pass: a chunk of zc planes takes zc + 4 plane
 // iterations (two-step: zc + 2) and three collisions per site become
@@ -697,10 +699,14 @@ constexpr int collide3_threads(int tx, int ty) { return tx * (ty + 4) + 64; }
 
 // Planes per workgroup of the three-step pass: as two_step_zc, with zc + 4
 // plane iterations per chunk.  0 when the launch would not reach half of the
-// CUs (unless L.two_step == 2) or the lattice is too thin in z.
+// CUs (unless L.two_step == 2) or the lattice is too thin in z (periodic:
+// fewer than 4 planes; slab: fewer than 12, whose pass covers planes
+// 3 .. nz-4).
+constexpr int SLAB3_MIN_NZ = 12;
 inline int three_step_zc(const Lattice &L, int tx, int ty, int zc_max) {
-    const int nz = L.nz;
-    if (L.layout != BGK_LAYOUT_PERIODIC || nz < 4) return 0;
+    const bool slab = L.layout != BGK_LAYOUT_PERIODIC;
+    if (L.nz < (slab ? SLAB3_MIN_NZ : 4)) return 0;
+    const int nz = slab ? L.nz - 6 : L.nz;
     const long long txy = (long long)((L.nx + tx - 1) / tx) * ((L.ny + ty - 1) / ty);
     const long long ncu = L.ncu > 0 ? L.ncu : 1;
     const int lo = nz < TWO_STEP_MIN_ZC ? nz : TWO_STEP_MIN_ZC;
@@ -736,8 +742,8 @@ __global__ __launch_bounds__(collide3_threads(TX, TY), 1) void k_collide3(Pops<S
     const int tx = t % g.tiles_x;
     const int r = t / g.tiles_x;
     const int ty = r % g.tiles_y;
-    const int za = (r / g.tiles_y) * g.zc;
-    const int zb = min(g.nz, za + g.zc);
+    const int za = g.zlo + (r / g.tiles_y) * g.zc;
+    const int zb = min(g.zhi, za + g.zc);
 
     const int tid = (int)threadIdx.x;
     int ex, ey;
@@ -768,15 +774,18 @@ __global__ __launch_bounds__(collide3_threads(TX, TY), 1) void k_collide3(Pops<S
     const uint32_t Y[3] = {(uint32_t)ym * ys, (uint32_t)gy * ys, (uint32_t)yp * ys};
     const ptrdiff_t plane = (ptrdiff_t)g.zstride;
 
-    auto wrapz = [&](int z) { return z < 0 ? z + g.nz : (z >= g.nz ? z - g.nz : z); };
+    // z in [zlo-2, zhi+1]; a slab's planes need no wrap
+    auto wrapz = [&](int z) {
+        return !g.periodic_z ? z : (z < 0 ? z + g.nz : (z >= g.nz ? z - g.nz : z));
+    };
     auto load_plane = [&](int z, S(&raw)[19]) {
         const int p = wrapz(z);
-        const int zm = (p == 0) ? g.nz - 1 : p - 1;
-        const int zp = (p == g.nz - 1) ? 0 : p + 1;
+        const int zm = !g.periodic_z ? p - 1 : ((p == 0) ? g.nz - 1 : p - 1);
+        const int zp = !g.periodic_z ? p + 1 : ((p == g.nz - 1) ? 0 : p + 1);
         const S *b0 = src.p[0];
-        const __amdgpu_buffer_rsrc_t R[3] = {raw_rsrc(b0 + (ptrdiff_t)zm * plane),
-                                             raw_rsrc(b0 + (ptrdiff_t)p * plane),
-                                             raw_rsrc(b0 + (ptrdiff_t)zp * plane)};
+        const __amdgpu_buffer_rsrc_t R[3] = {raw_rsrc(b0 + (ptrdiff_t)(zm + g.zoff) * plane),
+                                             raw_rsrc(b0 + (ptrdiff_t)(p + g.zoff) * plane),
+                                             raw_rsrc(b0 + (ptrdiff_t)(zp + g.zoff) * plane)};
 #pragma unroll
         for (int q = 0; q < 19; q++)
             raw[q] = buf_ld<S>(R[1 - cz(q)], X[1 - cx(q)] + Y[1 - cy(q)], (uint32_t)q * g.pstride);
@@ -844,7 +853,8 @@ __global__ __launch_bounds__(collide3_threads(TX, TY), 1) void k_collide3(Pops<S
                 f3[q] = Storage<S>::template load<C>(
                     pull(l2, q, ey - 1, ex - 1, pb, s2[QN], r2O_2, r2R_1));
             collide_site<C, SH>(f3, b, omega, cte1, ux, uy, uz);
-            const __amdgpu_buffer_rsrc_t R = raw_rsrc(dst.p[0] + (ptrdiff_t)wrapz(z - 2) * plane);
+            const __amdgpu_buffer_rsrc_t R =
+                raw_rsrc(dst.p[0] + (ptrdiff_t)(wrapz(z - 2) + g.zoff) * plane);
 #pragma unroll
             for (int q = 0; q < 19; q++)
                 buf_st<S>(Storage<S>::template store<C>(b[q]), R, X[1] + Y[1],

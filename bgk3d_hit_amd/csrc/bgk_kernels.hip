@@ -5,7 +5,7 @@
 //                      in the index math (or slab ghost planes in z)
 //   k_collide2         two time steps per pass (launchers here, kernel in
 //                      bgk_collide.h); on the slab for the bulk planes
-//   k_collide3         three time steps per pass (periodic layout)
+//   k_collide3         three time steps per pass; on the slab for the bulk planes
 //   k_diagno           at-site moment sums (SRC/diagno.F90:73-115)
 //   k_dissipation      pulled Pi^neq / kinetic energy sums
 //                      (SRC/dissipation.F90:70-253)
@@ -75,15 +75,20 @@ static hipError_t collide2_t(const Lattice &L, int src, double omega, void *face
                                         : collide2_o<S, C, false>(L, src, omega, faces, s);
 }
 
-// three steps from copy `src` into copy 1-src (periodic lattice, 32-bit
-// offsets)
+// three steps from copy `src` into copy 1-src (periodic lattice, or the
+// bulk planes 3 .. nz-4 of a slab; 32-bit offsets)
 template <typename S, typename C, bool SH>
 static hipError_t collide3_sh(const Lattice &L, int src, double omega, hipStream_t s) {
     using T = Tile3<S, C>;
     const int zc = three_step_zc(L, T::X, T::Y, T::ZC);
     if (zc <= 0) return hipErrorInvalidValue;  // caller checks three_step_ok first
     int nblocks;
-    const Geo2 g = make_geo2(L, T::X, T::Y, zc, &nblocks);
+    Geo2 g = make_geo2(L, T::X, T::Y, zc, &nblocks);
+    if (!g.periodic_z) {  // slab: the bulk third-step planes 3 .. nz-4
+        g.zlo = 3;
+        g.zhi = L.nz - 3;
+        nblocks = g.tiles_x * g.tiles_y * ((g.zhi - g.zlo + zc - 1) / zc);
+    }
     hipLaunchKernelGGL((k_collide3<S, C, T::X, T::Y, SH>), dim3(nblocks),
                        dim3(collide3_threads(T::X, T::Y)), 0, s, pops_of<S>(L, src),
                        pops_of<S>(L, 1 - src), g, (C)omega, (C)(L.shifted ? 1.0 : 0.0));
@@ -719,7 +724,7 @@ static bool three_step_ok_t(const Lattice &L) {
     return offsets_fit_32(L, sizeof(S)) && three_step_zc(L, T::X, T::Y, T::ZC) > 0;
 }
 bool three_step_ok(const Lattice &L) {
-    if (!L.two_step || !L.three_step || L.layout != BGK_LAYOUT_PERIODIC) return false;
+    if (!L.two_step || !L.three_step) return false;
     switch (L.storage) {
     case BGK_STORE_F64: return three_step_ok_t<double, double>(L);
     case BGK_STORE_F32: return three_step_ok_t<float, float>(L);

@@ -171,7 +171,7 @@ def test_slab_two_step_ragged(gpu, kind, shape):
     u = bgk.hit_velocity(*shape, 2.0, 0.03, 13)
     p = bgk.Lattice(*shape, omega=1.93, storage=kind, two_step=False)
     s = bgk.Lattice(*shape, omega=1.93, storage=kind, layout="slab", two_step="always",
-                    timing=True)
+                    timing=True, three_step=False)
     p.init_equilibrium(u)
     s.init_equilibrium(u)
     for n in (2, 3, 1, 6):

@@ -120,3 +120,49 @@ def test_step_split_into_passes(gpu, n, depths):
     g.step(n)
     t = g.timing_passes()
     assert (t[1][1], t[2][1], t[3][1]) == depths
+
+
+@pytest.mark.parametrize("kind", KINDS3)
+@pytest.mark.parametrize("shape", [(70, 9, 12), (33, 20, 13), (64, 16, 40), (130, 17, 29)])
+def test_slab_three_step_equals_single_steps(gpu, kind, shape):
+    """One rank's slab: the bulk three-step pass (planes 3..nz-4) with the
+    boundary planes stepped three times through two plane buffers, one
+    exchange per step -- identical to periodic single steps, at the thinnest
+    slab it takes (12 planes), odd plane counts and ragged x/y."""
+    say("slab three-step", kind, shape)
+    u = bgk.hit_velocity(*shape, 2.0, 0.03, 17)
+    p = bgk.Lattice(*shape, omega=1.93, storage=kind, two_step=False)
+    s = bgk.Lattice(*shape, omega=1.93, storage=kind, layout="slab", two_step="always",
+                    timing=True)
+    p.init_equilibrium(u)
+    s.init_equilibrium(u)
+    for n in (3, 1, 5, 6, 4):
+        p.step(n)
+        s.step(n)
+        assert np.array_equal(s.get_populations(), p.get_populations()), n
+    t = s.timing_passes()
+    assert (t[2][1], t[3][1]) == (3, 4)
+
+
+@pytest.mark.parametrize("kind", ["f64", "f32", "f16"])
+@pytest.mark.parametrize("three", [True, False])
+def test_slab_ghosts_after_fused_passes(gpu, kind, three):
+    """The slab's reference-compatible dissipation reads the ghost planes as
+    the exchanges left them (images of the state two steps back): after
+    fused passes, mixed with single steps, it equals the single-step run's
+    bit for bit."""
+    say("slab ghosts", kind, three)
+    shape = (40, 24, 16)
+    nu = bgk.svisc_from_omega(1.93)
+    u = bgk.hit_velocity(*shape, 2.0, 0.03, 23)
+    a = bgk.Lattice(*shape, omega=1.93, storage=kind, layout="slab", two_step="always",
+                    three_step=three)
+    b = bgk.Lattice(*shape, omega=1.93, storage=kind, layout="slab", two_step=False)
+    a.init_equilibrium(u)
+    b.init_equilibrium(u)
+    for n in (3, 1, 2, 1, 4, 1, 5, 2, 3):
+        a.step(n)
+        b.step(n)
+        assert np.array_equal(a.get_populations(), b.get_populations()), n
+        assert np.array_equal(a.dissipation(nu, refcompat=True),
+                              b.dissipation(nu, refcompat=True)), n
