@@ -139,15 +139,16 @@ int bgk_col_MC_masked(bgk_ctx *ctx, int opt);
 
 /* nsteps of the time loop body boundaries + col_MC (SRC/bgk3d_HIT.F90:127-146
  * without diagnostic).  Where the lattice is large enough (unless
- * BGK_FLAG_SINGLE_STEP): three steps per pass over the populations (periodic
- * layout, no halo images; unless
- * BGK_FLAG_NO_THREE_STEP), then two steps per pass, then single steps;
- * bit-identical to nsteps calls of bgk_col_MC, stale halo images included.  Slab layout: the interior collide overlaps the halo
- * pack/exchange/unpack and boundary collide on a second HIP stream (the
- * structure of SRC/bcond.comm.step9.F90:349 and
- * SRC/do_somethingGPU_overlap.F90:48-58); with two steps per pass the
- * boundary planes take both steps through a plane buffer on that stream,
- * with one exchange per step as before.                                     */
+ * BGK_FLAG_SINGLE_STEP): three steps per pass over the populations (not with
+ * BGK_FLAG_REFCOMPAT_HALOS or BGK_FLAG_NO_THREE_STEP; slabs of 12 planes and
+ * more), two steps per pass for a remainder of two (two such passes for a
+ * remainder of one), single steps otherwise; bit-identical to nsteps calls
+ * of bgk_col_MC, stale halo images and slab ghost planes included.  Slab
+ * layout: the interior collide overlaps the halo pack/exchange/unpack and
+ * boundary collide on a second HIP stream (the structure of
+ * SRC/bcond.comm.step9.F90:349 and SRC/do_somethingGPU_overlap.F90:48-58);
+ * with n steps per pass the boundary planes take the n steps through plane
+ * buffers on that stream, with one exchange per step as before.            */
 int bgk_step(bgk_ctx *ctx, int nsteps);
 
 /* at-site moments of the current state: replaces diagno (SRC/diagno.F90:28-154).
