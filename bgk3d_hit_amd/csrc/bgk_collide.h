@@ -726,7 +726,9 @@ inline int three_step_zc(const Lattice &L, int tx, int ty, int zc_max) {
     return best;
 }
 
-template <typename S, typename C, int TX, int TY, bool SH>
+// SLAB: the slab's bulk (z offset g.zoff, no wrap), compiled apart so that
+// the periodic pass carries none of it
+template <typename S, typename C, int TX, int TY, bool SH, bool SLAB = false>
 __global__ __launch_bounds__(collide3_threads(TX, TY), 1) void k_collide3(Pops<S> src,
                                                                           Pops<S> dst, Geo2 g,
                                                                           C omega, C cte1) {
@@ -742,8 +744,8 @@ __global__ __launch_bounds__(collide3_threads(TX, TY), 1) void k_collide3(Pops<S
     const int tx = t % g.tiles_x;
     const int r = t / g.tiles_x;
     const int ty = r % g.tiles_y;
-    const int za = g.zlo + (r / g.tiles_y) * g.zc;
-    const int zb = min(g.zhi, za + g.zc);
+    const int za = (SLAB ? g.zlo : 0) + (r / g.tiles_y) * g.zc;
+    const int zb = min(SLAB ? g.zhi : g.nz, za + g.zc);
 
     const int tid = (int)threadIdx.x;
     int ex, ey;
@@ -775,17 +777,18 @@ __global__ __launch_bounds__(collide3_threads(TX, TY), 1) void k_collide3(Pops<S
     const ptrdiff_t plane = (ptrdiff_t)g.zstride;
 
     // z in [zlo-2, zhi+1]; a slab's planes need no wrap
+    const int zoff = SLAB ? g.zoff : 0;
     auto wrapz = [&](int z) {
-        return !g.periodic_z ? z : (z < 0 ? z + g.nz : (z >= g.nz ? z - g.nz : z));
+        return SLAB ? z : (z < 0 ? z + g.nz : (z >= g.nz ? z - g.nz : z));
     };
     auto load_plane = [&](int z, S(&raw)[19]) {
         const int p = wrapz(z);
-        const int zm = !g.periodic_z ? p - 1 : ((p == 0) ? g.nz - 1 : p - 1);
-        const int zp = !g.periodic_z ? p + 1 : ((p == g.nz - 1) ? 0 : p + 1);
+        const int zm = SLAB ? p - 1 : ((p == 0) ? g.nz - 1 : p - 1);
+        const int zp = SLAB ? p + 1 : ((p == g.nz - 1) ? 0 : p + 1);
         const S *b0 = src.p[0];
-        const __amdgpu_buffer_rsrc_t R[3] = {raw_rsrc(b0 + (ptrdiff_t)(zm + g.zoff) * plane),
-                                             raw_rsrc(b0 + (ptrdiff_t)(p + g.zoff) * plane),
-                                             raw_rsrc(b0 + (ptrdiff_t)(zp + g.zoff) * plane)};
+        const __amdgpu_buffer_rsrc_t R[3] = {raw_rsrc(b0 + (ptrdiff_t)(zm + zoff) * plane),
+                                             raw_rsrc(b0 + (ptrdiff_t)(p + zoff) * plane),
+                                             raw_rsrc(b0 + (ptrdiff_t)(zp + zoff) * plane)};
 #pragma unroll
         for (int q = 0; q < 19; q++)
             raw[q] = buf_ld<S>(R[1 - cz(q)], X[1 - cx(q)] + Y[1 - cy(q)], (uint32_t)q * g.pstride);
@@ -854,7 +857,7 @@ __global__ __launch_bounds__(collide3_threads(TX, TY), 1) void k_collide3(Pops<S
                     pull(l2, q, ey - 1, ex - 1, pb, s2[QN], r2O_2, r2R_1));
             collide_site<C, SH>(f3, b, omega, cte1, ux, uy, uz);
             const __amdgpu_buffer_rsrc_t R =
-                raw_rsrc(dst.p[0] + (ptrdiff_t)(wrapz(z - 2) + g.zoff) * plane);
+                raw_rsrc(dst.p[0] + (ptrdiff_t)(wrapz(z - 2) + zoff) * plane);
 #pragma unroll
             for (int q = 0; q < 19; q++)
                 buf_st<S>(Storage<S>::template store<C>(b[q]), R, X[1] + Y[1],

@@ -84,14 +84,19 @@ static hipError_t collide3_sh(const Lattice &L, int src, double omega, hipStream
     if (zc <= 0) return hipErrorInvalidValue;  // caller checks three_step_ok first
     int nblocks;
     Geo2 g = make_geo2(L, T::X, T::Y, zc, &nblocks);
-    if (!g.periodic_z) {  // slab: the bulk third-step planes 3 .. nz-4
+    const C w = (C)omega, cte1 = (C)(L.shifted ? 1.0 : 0.0);
+    if (g.periodic_z) {
+        hipLaunchKernelGGL((k_collide3<S, C, T::X, T::Y, SH>), dim3(nblocks),
+                           dim3(collide3_threads(T::X, T::Y)), 0, s, pops_of<S>(L, src),
+                           pops_of<S>(L, 1 - src), g, w, cte1);
+    } else {  // slab: the bulk third-step planes 3 .. nz-4
         g.zlo = 3;
         g.zhi = L.nz - 3;
         nblocks = g.tiles_x * g.tiles_y * ((g.zhi - g.zlo + zc - 1) / zc);
+        hipLaunchKernelGGL((k_collide3<S, C, T::X, T::Y, SH, true>), dim3(nblocks),
+                           dim3(collide3_threads(T::X, T::Y)), 0, s, pops_of<S>(L, src),
+                           pops_of<S>(L, 1 - src), g, w, cte1);
     }
-    hipLaunchKernelGGL((k_collide3<S, C, T::X, T::Y, SH>), dim3(nblocks),
-                       dim3(collide3_threads(T::X, T::Y)), 0, s, pops_of<S>(L, src),
-                       pops_of<S>(L, 1 - src), g, (C)omega, (C)(L.shifted ? 1.0 : 0.0));
     return hipGetLastError();
 }
 template <typename S, typename C>
