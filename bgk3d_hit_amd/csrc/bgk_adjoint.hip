@@ -178,7 +178,7 @@ __global__ __launch_bounds__(collide2_threads(TX, TY), MINB) void k_adjoint2(Pop
                                                                              Pops<C> dst, Geo2 g,
                                                                              AdjointOp<C> op) {
     __shared__ C lds[20][TY + 2][TX + 2];
-    two_step_pass<C, C, O32, TX, TY, AdjointOp<C>, false, O32>(src, dst, g, op, lds);
+    two_step_pass<C, C, O32, TX, TY, AdjointOp<C>, false, O32, 1>(src, dst, g, op, lds);
 }
 
 // dJ/du0 = sum_i lambda^0_i w_i (3 c_i + 9 (c_i.u0) c_i - 3 u0),
