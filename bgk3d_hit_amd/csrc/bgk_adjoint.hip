@@ -181,6 +181,53 @@ __global__ __launch_bounds__(collide2_threads(TX, TY), MINB) void k_adjoint2(Pop
     two_step_pass<C, C, O32, TX, TY, AdjointOp<C>, false, O32, 1>(src, dst, g, op, lds);
 }
 
+// Three adjoint steps in one pass (three_step_pass with the pull mirrored):
+// M^{t+1} -> M^t (u^t from tape0) -> M^{t-1} (tape1), both kept in LDS ->
+// M^{t-2} (tape2).  Bit-identical to three k_adjoint_step launches.
+template <typename C>
+struct AdjointOp3 {
+    static constexpr int SGN = -1;
+    static constexpr bool PINGPONG = true;
+    struct Extra {
+        C u[3];
+    };
+    C omega;
+    const C *tape0, *tape1, *tape2;
+    size_t ld;
+    __device__ __forceinline__ Extra at(const C *t, size_t i) const {
+        return Extra{{t[i], t[ld + i], t[2 * ld + i]}};
+    }
+    __device__ __forceinline__ Extra extra_first(size_t i) const { return at(tape0, i); }
+    __device__ __forceinline__ Extra extra_second(size_t i) const { return at(tape1, i); }
+    __device__ __forceinline__ Extra extra_third(size_t i) const { return at(tape2, i); }
+    __device__ __forceinline__ void first(const C (&f)[19], C (&b)[19], const Extra &e, size_t,
+                                          bool) const {
+        adjoint_site<C>(f, e.u[0], e.u[1], e.u[2], omega, b);
+    }
+    __device__ __forceinline__ void second(const C (&f)[19], C (&b)[19], const Extra &e, size_t,
+                                           bool) const {
+        adjoint_site<C>(f, e.u[0], e.u[1], e.u[2], omega, b);
+    }
+    __device__ __forceinline__ void third(const C (&f)[19], C (&b)[19], const Extra &e,
+                                          size_t) const {
+        adjoint_site<C>(f, e.u[0], e.u[1], e.u[2], omega, b);
+    }
+};
+
+template <typename C, int TX, int TY>
+__global__ __launch_bounds__(collide3_threads(TX, TY), 1) void k_adjoint3(Pops<C> src,
+                                                                          Pops<C> dst, Geo2 g,
+                                                                          AdjointOp3<C> op) {
+    three_step_pass<C, C, TX, TY, AdjointOp3<C>>(src, dst, g, op);
+}
+
+// three taped forward steps in one pass (u of each step to its tape slice)
+template <typename S, typename C, int TX, int TY>
+__global__ __launch_bounds__(collide3_threads(TX, TY), 1) void k_tape3(
+    Pops<S> src, Pops<S> dst, Geo2 g, CollideOp3<C, true, true> op) {
+    three_step_pass<S, C, TX, TY, CollideOp3<C, true, true>>(src, dst, g, op);
+}
+
 // dJ/du0 = sum_i lambda^0_i w_i (3 c_i + 9 (c_i.u0) c_i - 3 u0),
 // lambda^0_i(x) = M^0_i(x + c_i)
 template <typename C>
@@ -280,12 +327,49 @@ hipError_t launch_tape2_o(bgk_ctx *c, C *tape0, C *tape1) {
     return hipGetLastError();
 }
 
+// three taped forward steps (slices tape0..tape2) in one pass
+template <typename S, typename C>
+hipError_t launch_tape3(bgk_ctx *c, C *tape0, C *tape1, C *tape2) {
+    const Lattice &L = c->lat;
+    using T = Tile3<S, C>;
+    int nblocks;
+    const Geo2 g = make_geo2(L, T::X, T::Y, three_step_zc(L, T::X, T::Y, T::ZC), &nblocks);
+    const CollideOp3<C, true, true> op{(C)c->omega, (C)1, tape0, tape1, tape2, (size_t)L.nsite};
+    hipLaunchKernelGGL((k_tape3<S, C, T::X, T::Y>), dim3(nblocks),
+                       dim3(collide3_threads(T::X, T::Y)), 0, c->s0, pops_of<S>(L, c->cur),
+                       pops_of<S>(L, 1 - c->cur), g, op);
+    c->cur ^= 1;
+    return hipGetLastError();
+}
+
+// Three-step passes of the adjoint's forward and backward: where
+// three_step_ok holds, lambda (in the compute kind) takes 32-bit offsets and
+// a pass has a full round of workgroups (one per CU; at 128^3, 208
+// workgroups, the adjoint was 2 % slower with them, at 256^3 the backward
+// 10 % faster).  Of n steps, n3 such passes (a remainder of one goes to two
+// two-step passes, as in bgk_step).
+template <typename C>
+int adjoint_triples(const Lattice &L, int n) {
+    if (!three_step_ok(L) || !offsets_fit_32(L, sizeof(C))) return 0;
+    using T = Tile3<C, C>;
+    const int zc = three_step_zc(L, T::X, T::Y, T::ZC);
+    const long long nb = (long long)((L.nx + T::X - 1) / T::X) * ((L.ny + T::Y - 1) / T::Y) *
+                         ((L.nz + zc - 1) / zc);
+    if (L.two_step != 2 && nb < L.ncu) return 0;
+    int n3 = n / 3;
+    if (n % 3 == 1 && n3 > 0 && two_step_ok(L)) n3--;
+    return n3;
+}
+
 // taped forward steps: slice k of `tape` records step k of the n steps
 template <typename S, typename C>
 int run_taped(bgk_ctx *c, C *tape, int n) {
     const Lattice &L = c->lat;
     const size_t slice = (size_t)3 * L.nsite;
     int k = 0;
+    for (int n3 = adjoint_triples<C>(L, n); n3 > 0; n3--, k += 3)
+        BGK_HIP((launch_tape3<S, C>(c, tape + k * slice, tape + (k + 1) * slice,
+                                    tape + (k + 2) * slice)));
     if (two_step_ok(L)) {
         const bool o32 = offsets_fit_32(L, sizeof(S));
         for (; k + 1 < n; k += 2)
@@ -318,6 +402,26 @@ hipError_t launch_adjoint2_o(bgk_ctx *c, const C *lin, C *lout, const C *tape0,
 }
 
 template <typename C>
+hipError_t launch_adjoint3(bgk_ctx *c, const C *lin, C *lout, const C *tape0, const C *tape1,
+                           const C *tape2) {
+    const Lattice &L = c->lat;
+    using T = Tile3<C, C>;
+    int nblocks;
+    Geo2 g = make_geo2(L, T::X, T::Y, three_step_zc(L, T::X, T::Y, T::ZC), &nblocks);
+    // lambda: the lattice's interleaved layout in the compute kind
+    g.pstride = (unsigned)(L.nx * sizeof(C));
+    Pops<C> a, b;
+    for (int q = 0; q < 19; q++) {
+        a.p[q] = (C *)lin + (size_t)q * L.nx;
+        b.p[q] = lout + (size_t)q * L.nx;
+    }
+    const AdjointOp3<C> op{(C)c->omega, tape0, tape1, tape2, (size_t)L.nsite};
+    hipLaunchKernelGGL((k_adjoint3<C, T::X, T::Y>), dim3(nblocks),
+                       dim3(collide3_threads(T::X, T::Y)), 0, c->s0, a, b, g, op);
+    return hipGetLastError();
+}
+
+template <typename C>
 hipError_t launch_adjoint_step(bgk_ctx *c, const C *lin, C *lout, const C *tape);
 
 // adjoint steps n-1 .. 0 of a segment whose tape slice k is step k; lin
@@ -327,6 +431,11 @@ int adjoint_sweep(bgk_ctx *c, C *&lin, C *&lout, const C *tape, int n) {
     const Lattice &L = c->lat;
     const size_t slice = (size_t)3 * L.nsite;
     int k = n - 1;
+    for (int n3 = adjoint_triples<C>(L, n); n3 > 0; n3--, k -= 3) {
+        BGK_HIP(launch_adjoint3<C>(c, lin, lout, tape + k * slice, tape + (k - 1) * slice,
+                                   tape + (k - 2) * slice));
+        std::swap(lin, lout);
+    }
     if (two_step_ok(L)) {
         const bool o32 = offsets_fit_32(L, sizeof(C));
         for (; k >= 1; k -= 2) {

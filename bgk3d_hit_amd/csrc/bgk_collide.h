@@ -904,4 +904,229 @@ __global__ __launch_bounds__(collide3_threads(TX, TY), 1) void k_collide3(Pops<S
     if (it < nit) iteration(it, ra, rb);
 }
 
+// ------------------------------------ three steps per pass, per-site operation
+// The three-step pass over a per-site operation (as two_step_pass): the
+// adjoint's taping forward and its backward sweep.  k_collide3 keeps its own
+// copy of this body: compiled through this generic form, its code changed
+// enough to cost 3.5 % in BF16 and 1.5 % in FP16 (measured), so the two are
+// kept apart and tested against each other bit for bit.
+
+// first, second and third run at S1, S2 and S3; own = the site is one of the
+// workgroup's own sites of its chunk.  TAPE: record u = j/rho of the three
+// steps at the own sites (slices tape0..tape2).
+template <typename C, bool TAPE, bool SH>
+struct CollideOp3 {
+    static constexpr int SGN = 1;
+    static constexpr bool PINGPONG = !TAPE;
+    struct Extra {};
+    C omega, cte1;
+    C *tape0, *tape1, *tape2;
+    size_t ld;  // sites per tape slice component
+    __device__ __forceinline__ Extra extra_first(size_t) const { return {}; }
+    __device__ __forceinline__ Extra extra_second(size_t) const { return {}; }
+    __device__ __forceinline__ Extra extra_third(size_t) const { return {}; }
+    __device__ __forceinline__ void tape(C *t, size_t i, C ux, C uy, C uz) const {
+        t[i] = ux;
+        t[ld + i] = uy;
+        t[2 * ld + i] = uz;
+    }
+    __device__ __forceinline__ void first(const C (&f)[19], C (&b)[19], const Extra &, size_t i,
+                                          bool own) const {
+        C ux, uy, uz;
+        collide_site<C, SH>(f, b, omega, cte1, ux, uy, uz);
+        if (TAPE && own) tape(tape0, i, ux, uy, uz);
+    }
+    __device__ __forceinline__ void second(const C (&f)[19], C (&b)[19], const Extra &, size_t i,
+                                           bool own) const {
+        C ux, uy, uz;
+        collide_site<C, SH>(f, b, omega, cte1, ux, uy, uz);
+        if (TAPE && own) tape(tape1, i, ux, uy, uz);
+    }
+    __device__ __forceinline__ void third(const C (&f)[19], C (&b)[19], const Extra &,
+                                          size_t i) const {
+        C ux, uy, uz;
+        collide_site<C, SH>(f, b, omega, cte1, ux, uy, uz);
+        if (TAPE) tape(tape2, i, ux, uy, uz);
+    }
+};
+
+// Periodic lattice, 32-bit offsets through one buffer resource per z plane
+// (populations g.pstride bytes apart).  Op::SGN = -1 mirrors the pull (the
+// adjoint reads x + c_q).  Every thread runs the same iterations and barriers.
+template <typename S, typename C, int TX, int TY, class Op>
+__device__ __forceinline__ void three_step_pass(const Pops<S> &src, const Pops<S> &dst,
+                                                const Geo2 &g, const Op &op) {
+    using X_t = typename Op::Extra;
+    constexpr int SGN = Op::SGN;
+    constexpr int EX = TX + 4, EY = TY + 4;
+    constexpr int NMAIN = TX * EY;
+    static_assert(4 * EY <= 64, "the edge wave covers four grown columns");
+    constexpr uint32_t es = sizeof(S);
+    // the z-only populations: QN read from the newest plane, QO from two
+    // planes before, QR the rest population
+    constexpr int QN = SGN > 0 ? 14 : 5, QO = SGN > 0 ? 5 : 14, QR = 18;
+    __shared__ S l1[20][EY][EX];          // S1, site (ey, ex)
+    __shared__ S l2[20][EY - 2][EX - 2];  // S2, site (ey, ex) at [ey-1][ex-1]
+
+    const int t = xcd_round_remap((int)blockIdx.x, (int)gridDim.x, g.xcd_group);
+    const int tx = t % g.tiles_x;
+    const int r = t / g.tiles_x;
+    const int ty = r % g.tiles_y;
+    const int za = (r / g.tiles_y) * g.zc;
+    const int zb = min(g.nz, za + g.zc);
+
+    const int tid = (int)threadIdx.x;
+    int ex, ey;
+    bool active;
+    if (tid < NMAIN) {
+        ex = tid % TX + 2;
+        ey = tid / TX;
+        active = true;
+    } else {
+        const int e = tid - NMAIN;
+        const int col = e / EY;  // grown columns 0, 1, EX-2, EX-1
+        active = e < 4 * EY;
+        ex = col < 2 ? col : EX - 4 + col;
+        ey = e % EY;
+    }
+    const int lx = tx * TX + ex - 2, ly = ty * TY + ey - 2;  // may leave the box
+    const int gx = ((lx % g.nx) + g.nx) % g.nx;
+    const int gy = ((ly % g.ny) + g.ny) % g.ny;
+    const bool mid = active && ex >= 1 && ex <= EX - 2 && ey >= 1 && ey <= EY - 2;
+    const bool owner = tid < NMAIN && ey >= 2 && ey <= EY - 3 && lx < g.nx && ly < g.ny;
+
+    const int xm = (gx == 0) ? g.nx - 1 : gx - 1;
+    const int xp = (gx == g.nx - 1) ? 0 : gx + 1;
+    const int ym = (gy == 0) ? g.ny - 1 : gy - 1;
+    const int yp = (gy == g.ny - 1) ? 0 : gy + 1;
+    const uint32_t X[3] = {(uint32_t)xm * es, (uint32_t)gx * es, (uint32_t)xp * es};
+    const uint32_t ys = (uint32_t)g.ystride * es;
+    const uint32_t Y[3] = {(uint32_t)ym * ys, (uint32_t)gy * ys, (uint32_t)yp * ys};
+    const ptrdiff_t plane = (ptrdiff_t)g.zstride;
+
+    auto wrapz = [&](int z) { return z < 0 ? z + g.nz : (z >= g.nz ? z - g.nz : z); };
+    // compact index of the column's site at plane p (tape, adjoint inputs)
+    auto site_of = [&](int p) {
+        return (size_t)gx + (size_t)g.nx * gy + (size_t)g.nxy * p;
+    };
+    auto load_plane = [&](int z, S(&raw)[19], X_t &xr) {
+        const int p = wrapz(z);
+        const int zm = (p == 0) ? g.nz - 1 : p - 1;
+        const int zp = (p == g.nz - 1) ? 0 : p + 1;
+        const S *b0 = src.p[0];
+        const __amdgpu_buffer_rsrc_t R[3] = {raw_rsrc(b0 + (ptrdiff_t)zm * plane),
+                                             raw_rsrc(b0 + (ptrdiff_t)p * plane),
+                                             raw_rsrc(b0 + (ptrdiff_t)zp * plane)};
+#pragma unroll
+        for (int q = 0; q < 19; q++)
+            raw[q] = buf_ld<S>(R[1 - SGN * cz(q)], X[1 - SGN * cx(q)] + Y[1 - SGN * cy(q)],
+                               (uint32_t)q * g.pstride);
+        xr = op.extra_first(site_of(p));
+    };
+    // pulled value of population q for the site (sy, sx) of a state whose
+    // x/y-moving populations are in `lds` (newest plane group 0, slot pb of
+    // the plane two before); the z-only ones from the column's registers
+    auto pull = [&](auto &lds, int q, int sy, int sx, int pb, S newest, S older, S rest) {
+        const int yy = sy - SGN * cy(q), xx = sx - SGN * cx(q);
+        if (pgroup<SGN>(q) == 0) return lds[pbase(0) + pslot<SGN>(q)][yy][xx];
+        if (pgroup<SGN>(q) == 1) return lds[pbase(1) + pslot<SGN>(q)][yy][xx];
+        if (pgroup<SGN>(q) == 2) return lds[pb + pslot<SGN>(q)][yy][xx];
+        return q == QO ? older : (q == QN ? newest : rest);
+    };
+
+    S r1O_1 = S{}, r1O_2 = S{}, r1R_1 = S{};  // S1 of this column at planes z-1, z-2
+    S r2O_1 = S{}, r2O_2 = S{}, r2R_1 = S{};  // S2 of this column, one plane behind
+    const int nit = zb - za + 4;              // iterations z = za-2 .. zb+1
+    auto iteration = [&](int it, S(&cur)[19], X_t &xcur, S(&nxt)[19], X_t &xnxt) {
+        const int z = za - 2 + it;
+        const int pb = pbase(2) + 4 * (it & 1);  // slot of S(z-2), then of S(z)
+        S s1[19], s2[19];
+        C f[19];
+#pragma unroll
+        for (int q = 0; q < 19; q++) {
+            f[q] = Storage<S>::template load<C>(cur[q]);
+            asm volatile("" : "+v"(f[q]));  // see two_step_pass
+        }
+        const X_t x1 = xcur;
+        if (it + 1 < nit) load_plane(z + 1, nxt, xnxt);
+        const bool do2 = mid && it >= 2, do3 = owner && it >= 4;
+        X_t x2{}, x3{};
+        if (do2) x2 = op.extra_second(site_of(wrapz(z - 1)));
+        if (do3) x3 = op.extra_third(site_of(wrapz(z - 2)));
+        if (active) {  // S1 at plane z
+            C b[19];
+            op.first(f, b, x1, site_of(wrapz(z)), owner && z >= za && z < zb);
+#pragma unroll
+            for (int q = 0; q < 19; q++) s1[q] = Storage<S>::template store<C>(b[q]);
+#pragma unroll
+            for (int q = 0; q < 19; q++)
+                if (pgroup<SGN>(q) == 0) l1[pbase(0) + pslot<SGN>(q)][ey][ex] = s1[q];
+        }
+        lds_barrier();
+        if (do2) {  // S2 at plane z-1
+            C f2[19], b[19];
+#pragma unroll
+            for (int q = 0; q < 19; q++)
+                f2[q] = Storage<S>::template load<C>(pull(l1, q, ey, ex, pb, s1[QN], r1O_2, r1R_1));
+            op.second(f2, b, x2, site_of(wrapz(z - 1)), owner && z - 1 >= za && z - 1 < zb);
+#pragma unroll
+            for (int q = 0; q < 19; q++) s2[q] = Storage<S>::template store<C>(b[q]);
+#pragma unroll
+            for (int q = 0; q < 19; q++)
+                if (pgroup<SGN>(q) == 0) l2[pbase(0) + pslot<SGN>(q)][ey - 1][ex - 1] = s2[q];
+        }
+        lds_barrier();
+        if (active) {
+#pragma unroll
+            for (int q = 0; q < 19; q++) {
+                if (pgroup<SGN>(q) == 1) l1[pbase(1) + pslot<SGN>(q)][ey][ex] = s1[q];
+                if (pgroup<SGN>(q) == 2) l1[pb + pslot<SGN>(q)][ey][ex] = s1[q];
+            }
+        }
+        if (do3) {  // S3 at plane z-2
+            C f3[19], b[19];
+#pragma unroll
+            for (int q = 0; q < 19; q++)
+                f3[q] = Storage<S>::template load<C>(
+                    pull(l2, q, ey - 1, ex - 1, pb, s2[QN], r2O_2, r2R_1));
+            op.third(f3, b, x3, site_of(wrapz(z - 2)));
+            const __amdgpu_buffer_rsrc_t R = raw_rsrc(dst.p[0] + (ptrdiff_t)wrapz(z - 2) * plane);
+#pragma unroll
+            for (int q = 0; q < 19; q++)
+                buf_st<S>(Storage<S>::template store<C>(b[q]), R, X[1] + Y[1],
+                          (uint32_t)q * g.pstride);
+        }
+        lds_barrier();
+        if (do2) {
+#pragma unroll
+            for (int q = 0; q < 19; q++) {
+                if (pgroup<SGN>(q) == 1) l2[pbase(1) + pslot<SGN>(q)][ey - 1][ex - 1] = s2[q];
+                if (pgroup<SGN>(q) == 2) l2[pb + pslot<SGN>(q)][ey - 1][ex - 1] = s2[q];
+            }
+            r2O_2 = r2O_1;
+            r2O_1 = s2[QO];
+            r2R_1 = s2[QR];
+        }
+        r1O_2 = r1O_1;
+        r1O_1 = s1[QO];
+        r1R_1 = s1[QR];
+    };
+
+    S ra[19];
+    X_t xa;
+    load_plane(za - 2, ra, xa);
+    if constexpr (Op::PINGPONG) {
+        S rb[19];
+        X_t xb;
+        int it = 0;
+        for (; it + 1 < nit; it += 2) {  // two register sets taking turns
+            iteration(it, ra, xa, rb, xb);
+            iteration(it + 1, rb, xb, ra, xa);
+        }
+        if (it < nit) iteration(it, ra, xa, rb, xb);
+    } else {
+        for (int it = 0; it < nit; it++) iteration(it, ra, xa, ra, xa);
+    }
+}
+
 }  // namespace bgk

@@ -35,15 +35,17 @@ def test_adjoint_f64_matches_oracle(gpu, levels, K):
 
 
 @pytest.mark.parametrize("kind", ["f64", "f32", "f32mix"])
-@pytest.mark.parametrize("levels,K", [(0, 1), (1, 7), (2, 3)])
-def test_adjoint_two_step_passes_equal_single_steps(gpu, kind, levels, K):
-    """Taped forward and adjoint sweep with two steps per pass (forced at a
-    small size, odd segment lengths so single steps follow fused pairs) give
-    the single-step gradient bit for bit."""
+@pytest.mark.parametrize("levels,K", [(0, 1), (1, 7), (2, 3), (1, 5)])
+@pytest.mark.parametrize("three", [True, False])
+def test_adjoint_two_step_passes_equal_single_steps(gpu, kind, levels, K, three):
+    """Taped forward and adjoint sweep with three and two steps per pass
+    (forced at a small size; segment lengths 1, 3, 5, 7 and a 31-step full
+    tape, so fused passes of both depths and single steps mix) give the
+    single-step gradient bit for bit."""
     shape, omega, T = (70, 9, 20), 1.9, 31
-    say("adjoint two-step vs single", kind, levels, K)
+    say("adjoint fused vs single", kind, levels, K, three)
     u0 = bgk.hit_velocity(*shape, 2.0, 0.03, 8)
-    a = bgk.Lattice(*shape, omega=omega, storage=kind, two_step="always")
+    a = bgk.Lattice(*shape, omega=omega, storage=kind, two_step="always", three_step=three)
     b = bgk.Lattice(*shape, omega=omega, storage=kind, two_step=False)
     Ja, ga, _ = a.adjoint_ke(u0, T, ckpt_every=K, levels=levels)
     Jb, gb, _ = b.adjoint_ke(u0, T, ckpt_every=K, levels=levels)
