@@ -170,7 +170,7 @@ def slab_config(steps, warmup):
     return out
 
 
-def c4_config(steps, warmup):
+def periodic_1024_config(steps, warmup):
     res = {}
     ref = None
     for kind in ("f32", "f16", "bf16"):
@@ -183,7 +183,7 @@ def c4_config(steps, warmup):
     return res
 
 
-def c5_config():
+def adjoint_128_config():
     from bgk3d_hit_amd.studies import adjoint_study
     r = adjoint_study(n=128, steps=500, ckpt=25, levels=1, fd=False)
     return {"workload": "128^3 FP32 HIT k0=4 u_rms=0.02 omega=1.90, 500-step horizon, "
@@ -394,9 +394,9 @@ def main():
         log("[bench] slab 1024x1024x256")
         configs["slab_1024x1024x256"] = guarded("slab", lambda: slab_config(100, 10))
         log("[bench] 1024^3 f32/f16/bf16")
-        configs["periodic_1024cubed"] = guarded("1024^3", lambda: c4_config(100, 10))
+        configs["periodic_1024cubed"] = guarded("1024^3", lambda: periodic_1024_config(100, 10))
         log("[bench] adjoint 128^3")
-        configs["adjoint_128cubed"] = guarded("adjoint", c5_config)
+        configs["adjoint_128cubed"] = guarded("adjoint", adjoint_128_config)
     cpu = None
     cpu_more = None
     if not a.no_cpu:

@@ -124,7 +124,7 @@ def test_torch_autograd_binding(gpu):
     assert res["grad_equal"]
 
 
-def test_config5_adjoint_128cubed_against_finite_differences(gpu):
+def test_adjoint_128cubed_500_steps_against_finite_differences(gpu):
     """128^3 FP32, k0=4, u_rms=0.02, omega=1.90, 500 steps, checkpoints
     every 25: gradient within 1e-4 of FP64 central differences (eps=1e-3)
     along 16 seeded solenoidal directions."""

@@ -87,7 +87,7 @@ def test_hit_trajectory_bitwise(gpu, kind, nsteps):
     assert np.array_equal(g.get_populations(), o.get_populations())
 
 
-def test_config1_64cubed_1000_steps_with_diagnostics(gpu):
+def test_hit_64cubed_fp64_1000_steps_with_diagnostics(gpu):
     """64^3 HIT, k0=4, urms=0.02, omega=1.90, FP64, 1000 steps, diagnostics
     every 100 steps in the reference loop order."""
     shape = (64, 64, 64)
@@ -391,7 +391,7 @@ def test_checksum_sees_one_ulp(gpu):
     assert g.checksum() == c0
 
 
-def test_config2_128cubed_fp64_against_oracle(gpu):
+def test_hit_128cubed_fp64_against_oracle(gpu):
     """128^3 HIT, omega=1.95, FP64: GPU against the CPU oracle, diagnostics
     every 50 steps."""
     shape, omega = (128, 128, 128), 1.95
@@ -410,7 +410,7 @@ def test_config2_128cubed_fp64_against_oracle(gpu):
     assert np.array_equal(g.get_populations(), o.get_populations())
 
 
-def test_config3_slab_1024x1024x256_bitwise(gpu):
+def test_slab_1024x1024x256_bitwise(gpu):
     """One rank's share at its real shape: 1024x1024x256 FP32 slab with
     +-z ghost planes and the two-stream overlap, bit-identical to the
     periodic kernel (k0=8, u_rms=0.03, omega=1.97)."""
