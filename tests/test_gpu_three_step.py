@@ -166,3 +166,27 @@ def test_slab_ghosts_after_fused_passes(gpu, kind, three):
         assert np.array_equal(a.get_populations(), b.get_populations()), n
         assert np.array_equal(a.dissipation(nu, refcompat=True),
                               b.dissipation(nu, refcompat=True)), n
+
+
+@pytest.mark.parametrize("kind", KINDS3)
+@pytest.mark.parametrize("layout", ["periodic", "slab"])
+def test_fused_passes_against_oracle(gpu, kind, layout):
+    """Three- and two-step passes (forced at a small size) against the CPU
+    oracle's single steps directly: 3 + 2 + 3 + 3 steps, populations bit for
+    bit."""
+    import oracle
+
+    say("fused vs oracle", kind, layout)
+    shape, omega = (40, 24, 16), 1.93
+    u = bgk.hit_velocity(*shape, 3.0, 0.03, 31)
+    g = bgk.Lattice(*shape, omega=omega, storage=kind, layout=layout, two_step="always",
+                    timing=True)
+    o = oracle.OracleLattice(*shape, omega=omega, kind=kind)
+    g.init_equilibrium(u)
+    o.init_equilibrium(u)
+    for n in (3, 2, 6):
+        g.step(n)
+        o.step(n)
+        assert np.array_equal(g.get_populations(), o.get_populations()), n
+    t = g.timing_passes()
+    assert (t[2][1], t[3][1]) == (1, 3)
