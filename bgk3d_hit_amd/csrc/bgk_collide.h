@@ -907,9 +907,11 @@ __global__ __launch_bounds__(collide3_threads(TX, TY), 1) void k_collide3(Pops<S
 // ------------------------------------ three steps per pass, per-site operation
 // The three-step pass over a per-site operation (as two_step_pass): the
 // adjoint's taping forward and its backward sweep.  k_collide3 keeps its own
-// copy of this body: compiled through this generic form, its code changed
-// enough to cost 3.5 % in BF16 and 1.5 % in FP16 (measured), so the two are
-// kept apart and tested against each other bit for bit.
+// copy of this body: compiled through a generic form its code changed (same
+// instruction counts, other register allocation and order), which cost 3.5 %
+// in BF16 with this form and 1.5 % in FP16 with LDS passed by pointer
+// (measured on one box against the kept kernel), so the two are kept apart;
+// both are tested against single steps bit for bit.
 
 // first, second and third run at S1, S2 and S3; own = the site is one of the
 // workgroup's own sites of its chunk.  TAPE: record u = j/rho of the three
