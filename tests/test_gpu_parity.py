@@ -412,20 +412,26 @@ def test_hit_128cubed_fp64_against_oracle(gpu):
 
 def test_slab_1024x1024x256_bitwise(gpu):
     """One rank's share at its real shape: 1024x1024x256 FP32 slab with
-    +-z ghost planes and the two-stream overlap, bit-identical to the
-    periodic kernel (k0=8, u_rms=0.03, omega=1.97)."""
+    +-z ghost planes and the two-stream overlap (three-step passes), over
+    the full 1000 steps of the workload (k0=8, u_rms=0.03, omega=1.97):
+    diagno every 50 steps and the final populations bit-identical to the
+    periodic lattice."""
     shape = (1024, 1024, 256)
-    say("slab 1024x1024x256")
+    say("slab 1024x1024x256 x 1000")
     p = bgk.Lattice(*shape, omega=1.97, storage="f32")
     p.init_hit(8.0, 0.03, 5678)
-    p.step(10)
-    cp, dp_ = p.checksum(), p.diagno()
+    dp_ = []
+    for _ in range(20):
+        p.step(50)
+        dp_.append(p.diagno())
+    cp = p.checksum()
     p.close()
     s = bgk.Lattice(*shape, omega=1.97, storage="f32", layout="slab")
     s.init_hit(8.0, 0.03, 5678)
-    s.step(10)
+    for k in range(20):
+        s.step(50)
+        assert np.array_equal(s.diagno(), dp_[k]), (k + 1) * 50
     assert s.checksum() == cp
-    assert np.array_equal(s.diagno(), dp_)
 
 
 def test_get_box_matches_full_download(gpu):
