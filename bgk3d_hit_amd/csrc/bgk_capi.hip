@@ -526,16 +526,19 @@ int bgk_create(int lx, int ly, int lz, int storage, int layout, int flags, doubl
         if ((e = hipMalloc(&c->halo_send, c->halo_bytes)) != hipSuccess)
             return bail(hip_fail(e, "hipMalloc(halo)"));
         c->bytes += c->halo_bytes;
-        if (L.two_step) {  // plane buffers of the fused boundary paths: 22
-                           // planes in the lattice's layout (same pitches)
+        if (L.two_step) {  // plane buffers of the fused boundary paths in the
+                           // lattice's layout (same pitches): 22 planes where
+                           // three-step passes may run, else 8
+            const bool three = L.three_step && L.nz >= SLAB3_MIN_NZ;
             const size_t pbytes = (size_t)L.zstride * es;
-            const size_t pb = (22 * pbytes + 255) & ~(size_t)255;
+            const size_t pb = ((three ? 22 : 8) * pbytes + 255) & ~(size_t)255;
             if ((e = hipMalloc(&c->slab_s1, pb)) != hipSuccess)
                 return bail(hip_fail(e, "hipMalloc(slab plane buffer)"));
             c->bytes += pb;
             for (int q = 0; q < 19; q++) {
                 c->slab_s1_pop[q] = (char *)c->slab_s1 + (size_t)q * lx * es;
-                c->slab_s2_pop[q] = (char *)c->slab_s1 + 12 * pbytes + (size_t)q * lx * es;
+                c->slab_s2_pop[q] =
+                    three ? (char *)c->slab_s1 + 12 * pbytes + (size_t)q * lx * es : nullptr;
             }
             if ((e = hipMemsetAsync(c->slab_s1, 0, pb, c->s0)) != hipSuccess)
                 return bail(hip_fail(e, "hipMemsetAsync(slab plane buffer)"));

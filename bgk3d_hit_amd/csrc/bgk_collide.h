@@ -717,7 +717,6 @@ constexpr int collide3_threads(int tx, int ty) { return tx * (ty + 4) + 64; }
 // CUs (unless L.two_step == 2) or the lattice is too thin in z (periodic:
 // fewer than 4 planes; slab: fewer than 12, whose pass covers planes
 // 3 .. nz-4).
-constexpr int SLAB3_MIN_NZ = 12;
 inline int three_step_zc(const Lattice &L, int tx, int ty, int zc_max) {
     const bool slab = L.layout != BGK_LAYOUT_PERIODIC;
     if (L.nz < (slab ? SLAB3_MIN_NZ : 4)) return 0;

@@ -102,11 +102,12 @@ struct bgk_ctx {
     // slab halo message: [face 0 = z+ (plane nz), 1 = z- (plane 1)][5][nx*ny]
     void *halo_send = nullptr;
     size_t halo_bytes = 0;
-    // slab fused boundary paths, one allocation of 22 planes in the
-    // lattice's layout: slab_s1_pop at planes 0..11 (two-step: first-step
-    // planes -1..2 and nz-3..nz at 0..7; three-step: -1..4 and nz-5..nz at
-    // 0..11), slab_s2_pop at planes 12..21 (three-step second step: -1..3 and
-    // nz-4..nz at 0..9)
+    // slab fused boundary paths, one allocation in the lattice's layout:
+    // slab_s1_pop at planes 0..11 (two-step: first-step planes -1..2 and
+    // nz-3..nz at 0..7; three-step: -1..4 and nz-5..nz at 0..11),
+    // slab_s2_pop at planes 12..21 (three-step second step: -1..3 and
+    // nz-4..nz at 0..9); 8 planes and no slab_s2_pop where three-step passes
+    // cannot run (BGK_FLAG_NO_THREE_STEP, fewer than 12 planes)
     void *slab_s1 = nullptr;
     void *slab_s1_pop[19] = {};  // plane buffer of the fused passes' first step
     void *slab_s2_pop[19] = {};  // of the three-step pass's second step
@@ -163,6 +164,8 @@ hipError_t launch_collide2(const Lattice &L, int src, double omega, void *faces,
 // three time steps in one pass from copy `src` into copy 1-src (periodic
 // layout; slab: the bulk planes 3 .. nz-4); only when three_step_ok(L)
 bool three_step_ok(const Lattice &L);
+// thinnest slab the three-step pass takes (its bulk covers planes 3..nz-4)
+constexpr int SLAB3_MIN_NZ = 12;
 hipError_t launch_collide3(const Lattice &L, int src, double omega, hipStream_t s);
 // diagno and dissipation in one pass: partials holds 2 * 8 * parts doubles,
 // out 16 (out[0..7] as launch_diagno's, out[8..15] as launch_dissipation's)
