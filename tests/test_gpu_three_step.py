@@ -24,7 +24,7 @@ def say(*a):
 @pytest.mark.parametrize("kind", KINDS3)
 @pytest.mark.parametrize("shifted", [True, False])
 @pytest.mark.parametrize("shape", [(70, 9, 6), (64, 16, 130), (24, 20, 16), (130, 17, 65),
-                                   (67, 23, 4), (5, 3, 7), (129, 31, 9)])
+                                   (67, 23, 4), (5, 3, 7), (129, 31, 9), (1, 1, 8), (3, 70, 4)])
 def test_three_step_pass_equals_single_steps(gpu, kind, shifted, shape):
     say("three-step vs single", kind, shifted, shape)
     u = bgk.hit_velocity(*shape, 2.0, 0.03, 13)
