@@ -302,10 +302,15 @@ subroutine diagnostic(itime)
   implicit none
   integer, intent(in) :: itime
   integer(8) :: c0, c1
+  real(c_double) :: ds(7)
   if (mod(itime, icheck) == 0) then
      call system_clock(c0)
-     call diagno(itime)
-     call dissipation(itime)
+     ! diagno and dissipation (SRC/diagnostic.F90:47-49) from one pass over
+     ! the populations, bit-identical to the two calls
+     call bgk_check(bgk_diagnostics(ctx, real(svisc, c_double), diag_c, ds), &
+          'bgk_diagnostics')
+     call write_diagno(itime, ds(1:5))
+     call write_dissipation(itime, ds(6:7))
      call probe_global(itime, lz / 2, ly / 2, lz / 2, 88)  ! SRC/diagnostic.F90:50
      if (mod(itime, ivtim) == 0) call vtk_slices(itime)
      call system_clock(c1)
@@ -323,8 +328,17 @@ subroutine diagno(itime)
   implicit none
   integer, intent(in) :: itime
   real(c_double) :: d(5)
-  logical, save :: first = .true.
   call bgk_check(bgk_diagno(ctx, diag_c, d), 'bgk_diagno')
+  call write_diagno(itime, d)
+end subroutine diagno
+
+! diagno's outputs (SRC/diagno.F90:132-152)
+subroutine write_diagno(itime, d)
+  use bgk_host
+  implicit none
+  integer, intent(in) :: itime
+  real(c_double), intent(in) :: d(5)
+  logical, save :: first = .true.
   if (first) then
      open(70, file='diagno.dat', status='unknown')
      open(72, file='diagno.full.dat', status='unknown')
@@ -343,7 +357,7 @@ subroutine diagno(itime)
 1002 format("       mean rho ", 1(e14.6, 1x))
 1003 format("       mean vel ", 4(e14.6, 1x))
 1004 format(i8, 5(e14.6, 1x))
-end subroutine diagno
+end subroutine write_diagno
 
 ! SRC/dissipation.F90:22-280
 subroutine dissipation(itime)
@@ -351,10 +365,19 @@ subroutine dissipation(itime)
   implicit none
   integer, intent(in) :: itime
   real(c_double) :: s(2)
+  call bgk_check(bgk_dissipation(ctx, real(svisc, c_double), diag_c, s), 'bgk_dissipation')
+  call write_dissipation(itime, s)
+end subroutine dissipation
+
+! dissipation's outputs (SRC/dissipation.F90:262-278)
+subroutine write_dissipation(itime, s)
+  use bgk_host
+  implicit none
+  integer, intent(in) :: itime
+  real(c_double), intent(in) :: s(2)
   real(sp) :: dx_s
   real(dp) :: dt, tke
   logical, save :: first = .true.
-  call bgk_check(bgk_dissipation(ctx, real(svisc, c_double), diag_c, s), 'bgk_dissipation')
   if (first) then
      open(71, file='dissipation.dat', status='unknown')
      open(73, file='dissipation.full.dat', status='unknown')
@@ -370,7 +393,7 @@ subroutine dissipation(itime)
   write(73, '(i8,4(1x,es24.16))') itime, real(itime, dp) * dt, s(2), s(1)
   flush(73)
 1004 format(i8, 4(e14.6, 1x))
-end subroutine dissipation
+end subroutine write_dissipation
 
 ! SRC/probe_global.F90:55-95: the 19 populations of global site (i0,j0,k0)
 subroutine probe_global(itime, ip, jp, kp, unit)
