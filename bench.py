@@ -15,14 +15,17 @@ alone and the one-pass-per-step kernel (k_collide) are reported beside it.
 
 `roofline` is measured live for the dominant kernel (the launch depth with
 the most collide time): HIP events around every collide launch on the stream
-it runs on give the average launch duration.  The unit of work is the
-site-update, whose algorithmic bytes are 19 loads + 19 stores (38 * s); a
-launch processes 512^3 site-updates per time step it advances (one, two or
-three), so `achieved` = 38 * s * 512^3 * steps per launch / launch time.
-The fused passes move each population through HBM once per two or three
-time steps, so their `achieved` can exceed the HBM peak: `frac` > 1 is the
-factor by which they beat the bandwidth bound of any one-step-per-launch
-kernel.  `traffic` is the HBM bytes per launch measured by rocprofv3
+it runs on give the average launch duration.  `achieved` counts the bytes
+one launch must move: every population of every site once in and once out,
+19 loads + 19 stores = 38 * s bytes per site, 38 * s * 512^3 per launch
+whether the launch advances one, two or three time steps (the fused passes
+keep their intermediate states on chip), so `frac` = achieved / 8 TB/s is
+at most 1.  `site_update_equivalent_gbs` counts the same launch per
+site-update (38 * s per site and time step, times the steps it advances):
+the rate a one-step-per-launch kernel would need to match it, which exceeds
+the HBM peak for the fused passes.  The reductions run every 50 steps and
+after the last timed step, so any --steps times at least one of them
+(`config.reductions_in_timed_region`).  `traffic` is the HBM bytes per launch measured by rocprofv3
 FETCH_SIZE / WRITE_SIZE runs of this script (profiles/collide_traffic.json,
 null when absent or for another lattice size); `hbm_actual_gbs` = traffic /
 launch time is the kernel's own DRAM rate, and `copy_ceiling_gbs` the rate
@@ -71,13 +74,15 @@ def run_gpu(kind, n, steps, warmup, diag_every, omega=1.95, shape=None, layout="
         diag = []
         t0 = time.perf_counter()
         done = 0
+        # the reductions run every diag_every steps and after the last
+        # step, so any step count times at least one of them:
+        # ceil(steps / diag_every) diagnostics passes in the timed region
         while done < steps:
             k = min(diag_every, steps - done)
             lat.step(k)
             done += k
-            if done % diag_every == 0:
-                d5, s2 = lat.diagnostics(nu)  # diagno + dissipation, one pass
-                diag.append((done + warmup, *d5, *s2))
+            d5, s2 = lat.diagnostics(nu)  # diagno + dissipation, one pass
+            diag.append((done + warmup, *d5, *s2))
         lat.synchronize()
         wall = time.perf_counter() - t0
         passes = lat.timing_passes(reset=False)
@@ -92,10 +97,16 @@ def run_gpu(kind, n, steps, warmup, diag_every, omega=1.95, shape=None, layout="
     # most collide time in the timed region
     spl = max(passes, key=lambda k: passes[k][0])
     avg_ms = passes[spl][0] / max(passes[spl][1], 1)
-    # algorithmic bytes of the site-updates one launch processes: 38 * s per
-    # site-update, nsite per time step, spl time steps per launch
-    algo_bytes = 38 * BYTES[kind] * nsite * spl
+    # algorithmic bytes one launch must move: every population of every site
+    # once in and once out, 38 * s per site, whatever the number of time
+    # steps the launch advances (the fused passes keep their intermediate
+    # states on chip)
+    algo_bytes = 38 * BYTES[kind] * nsite
     achieved = algo_bytes / (avg_ms * 1e-3) / 1e9
+    # the same launch counted per site-update (38 * s per site and time
+    # step): the rate a one-step-per-launch kernel would need to match it
+    su_bytes = algo_bytes * spl
+    su_gbs = su_bytes / (avg_ms * 1e-3) / 1e9
     return {
         "kind": kind,
         "kernel": KERNEL[spl] + f"<{kind}>",
@@ -112,6 +123,9 @@ def run_gpu(kind, n, steps, warmup, diag_every, omega=1.95, shape=None, layout="
         "achieved_gbs": achieved,
         "frac": achieved / HBM_PEAK_GBS,
         "algo_bytes_per_launch": algo_bytes,
+        "site_update_bytes_per_launch": su_bytes,
+        "site_update_equivalent_gbs": su_gbs,
+        "reductions_timed": len(diag),
         "nsite": nsite,
         "ke0": d0[4],
         "diag_last": diag[-1] if diag else None,
@@ -123,6 +137,7 @@ def summary(r, ref=None):
            "collide_ms_per_step": round(r["collide_ms_per_step"], 4),
            "kernel": r["kernel"], "steps_per_launch": r["steps_per_launch"],
            "achieved_gbs": round(r["achieved_gbs"], 1), "frac": round(r["frac"], 4),
+           "site_update_equivalent_gbs": round(r["site_update_equivalent_gbs"], 1),
            "storage_bytes": BYTES[r["kind"]], "compute": COMPUTE[r["kind"]],
            "traffic": load_traffic(r["kind"], r["steps_per_launch"], r["nsite"])}
     if out["traffic"]:
@@ -319,6 +334,10 @@ def roofline(r):
            "steps_per_launch": r["steps_per_launch"],
            "avg_launch_ms": round(r["collide_avg_ms"], 4),
            "algo_bytes_per_launch": r["algo_bytes_per_launch"],
+           "site_update_bytes_per_launch": r["site_update_bytes_per_launch"],
+           "site_update_equivalent_gbs": round(r["site_update_equivalent_gbs"], 1),
+           "site_update_equivalent_frac": round(r["site_update_equivalent_gbs"] / HBM_PEAK_GBS,
+                                                4),
            "hbm_actual_gbs": None, "hbm_actual_frac": None}
     if traffic:
         act = traffic / (r["collide_avg_ms"] * 1e-3) / 1e9
@@ -420,8 +439,10 @@ def main():
         "data": "synthetic: seeded HIT velocity (k0=4, u_rms=0.02, seed 1234), rho=1 "
                 "equilibrium populations",
         "config": {"workload": f"{a.size}^3 periodic D3Q19 BGK HIT, omega=1.95, FP32 storage "
-                               f"and collide, diagno+dissipation every {a.diag_every} steps",
+                               f"and collide, diagno+dissipation every {a.diag_every} steps "
+                               f"and after the last step",
                    "lattice": [a.size] * 3, "omega": 1.95, "diag_every": a.diag_every,
+                   "reductions_in_timed_region": main_run["reductions_timed"],
                    "layout": "per copy [z][y][q][x] (19 population rows of a y line side by "
                              "side), halo-free, periodic wrap in index math"},
         "roofline": roofline(main_run),
