@@ -24,6 +24,8 @@ FLAG_WIDE_OFFSETS = 32
 FLAG_NO_THREE_STEP = 64
 # diagnostic modes
 DIAG_CONSISTENT, DIAG_REFCOMPAT = 0, 1
+# reduction slots of bgk_diagnostics_noalloc's fixed scratch array
+NOALLOC_MAX_PARTS = 32768
 
 _c_int, _c_double, _p = ctypes.c_int, ctypes.c_double, ctypes.c_void_p
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -51,6 +53,7 @@ SIGNATURES = {
     "bgk_diagno": [_p, _c_int, _dp],
     "bgk_dissipation": [_p, _c_double, _c_int, _dp],
     "bgk_diagnostics": [_p, _c_double, _c_int, _dp],
+    "bgk_diagnostics_noalloc": [_p, _c_int, _c_int, _c_int, _c_int, _c_int, _c_double, _p, _p],
     "bgk_get_box": [_p, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _dp],
     "bgk_checksum": [_p, ctypes.POINTER(ctypes.c_uint64)],
     "bgk_synchronize": [_p],

@@ -417,6 +417,36 @@ static int check_ctx(bgk_ctx *c) {
 
 static const long long CHUNK = 1LL << 23;  // sites per host transfer chunk
 
+void lattice_geometry(Lattice &L, int lx, int ly, int lz, int storage, int layout, int flags,
+                      int device) {
+    L.ncu = 0;
+    if (hipDeviceGetAttribute(&L.ncu, hipDeviceAttributeMultiprocessorCount, device) !=
+            hipSuccess ||
+        L.ncu < 1)
+        L.ncu = 256;
+    const int nzs = layout == BGK_LAYOUT_SLAB_Z ? lz + 2 : lz;
+    L.nx = lx;
+    L.ny = ly;
+    L.nz = lz;
+    L.layout = layout;
+    L.storage = storage;
+    L.zoff = layout == BGK_LAYOUT_SLAB_Z ? 1 : 0;
+    L.nzs = nzs;
+    L.shifted = (flags & BGK_FLAG_NOSHIFT) ? 0 : 1;
+    L.nsite = (long long)lx * ly * lz;
+    L.plane_elems = (long long)lx * ly * nzs;
+    L.ystride = 19LL * lx;
+    L.zstride = 19LL * lx * ly;
+    L.copy_elems = 19LL * L.plane_elems;
+    L.wide_offsets = (flags & BGK_FLAG_WIDE_OFFSETS) ? 1 : 0;
+    L.two_step = !(flags & BGK_FLAG_SINGLE_STEP)
+                     ? ((flags & BGK_FLAG_TWO_STEP_ANY_SIZE) ? 2 : 1)
+                     : 0;
+    L.three_step = (flags & BGK_FLAG_NO_THREE_STEP) ? 0 : 1;
+    for (int k = 0; k < 2; k++)
+        for (int q = 0; q < 19; q++) L.pop[k][q] = nullptr;
+}
+
 }  // namespace bgk
 
 using namespace bgk;
@@ -466,25 +496,7 @@ int bgk_create(int lx, int ly, int lz, int storage, int layout, int flags, doubl
     bgk_ctx *c = new bgk_ctx();
     (void)hipGetDevice(&c->device);
     Lattice &L = c->lat;
-    L.ncu = 0;
-    if (hipDeviceGetAttribute(&L.ncu, hipDeviceAttributeMultiprocessorCount, c->device) !=
-            hipSuccess ||
-        L.ncu < 1)
-        L.ncu = 256;
-    L.nx = lx;
-    L.ny = ly;
-    L.nz = lz;
-    L.layout = layout;
-    L.storage = storage;
-    L.zoff = layout == BGK_LAYOUT_SLAB_Z ? 1 : 0;
-    L.nzs = nzs;
-    L.shifted = (flags & BGK_FLAG_NOSHIFT) ? 0 : 1;
-    L.nsite = (long long)lx * ly * lz;
-    L.plane_elems = plane;
-    L.ystride = 19LL * lx;
-    L.zstride = 19LL * lx * ly;
-    L.copy_elems = 19LL * plane;
-    L.wide_offsets = (flags & BGK_FLAG_WIDE_OFFSETS) ? 1 : 0;
+    lattice_geometry(L, lx, ly, lz, storage, layout, flags, c->device);
     c->flags = flags;
     c->omega = omega;
     c->timing.on = (flags & BGK_FLAG_TIMING) != 0;
@@ -496,10 +508,6 @@ int bgk_create(int lx, int ly, int lz, int storage, int layout, int flags, doubl
         bgk_destroy(c);
         return code;
     };
-    L.two_step = !(flags & BGK_FLAG_SINGLE_STEP)
-                     ? ((flags & BGK_FLAG_TWO_STEP_ANY_SIZE) ? 2 : 1)
-                     : 0;
-    L.three_step = (flags & BGK_FLAG_NO_THREE_STEP) ? 0 : 1;
     // two copies in the interleaved [z][y][q][x] layout (see Lattice)
     hipError_t e = hipMalloc(&c->pop_alloc, cbytes * 2);
     if (e != hipSuccess) return bail(hip_fail(e, "hipMalloc(populations)"));

@@ -143,6 +143,11 @@ int hip_fail(hipError_t e, const char *what);
 
 size_t storage_bytes(int storage);
 
+// sizes, pitches and pass options of a lattice (bgk_create's arguments);
+// the population pointers are left null for the caller to set
+void lattice_geometry(Lattice &L, int lx, int ly, int lz, int storage, int layout, int flags,
+                      int device);
+
 // timing helpers (no-ops unless BGK_FLAG_TIMING)
 // TCLS_PASS2 / TCLS_PASS3: collide launches of two / three time steps
 // (counted with the collide launches in bgk_timing)

@@ -169,6 +169,33 @@ int bgk_dissipation(bgk_ctx *ctx, double svisc, int mode, double out[2]);
  * reference routine does both.                                              */
 int bgk_diagnostics(bgk_ctx *ctx, double svisc, int mode, double out[7]);
 
+/* Allocation-free variant of bgk_diagnostics.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  It allocates nothing, synchronizes nothing and needs no
+ * context: the caller passes one copy of populations in device memory and a
+ * device array out[7], and the pass is queued on `stream` (a hipStream_t,
+ * NULL for the null stream).
+ *   pop     : one population copy of the periodic layout, storage kind
+ *             `storage`, as a context holds it: population q of site
+ *             (x, y, z) is element x + 19*lx*y + 19*lx*ly*z + q*lx
+ *             (rows of the 19 populations of each y line side by side)
+ *   flags   : 0 (shifted storage f_i - w_i) or BGK_FLAG_NOSHIFT
+ *   out     : device doubles, out[0..4] as bgk_diagno's (consistent mode),
+ *             out[5..6] as bgk_dissipation's; bit-identical to
+ *             bgk_diagnostics(ctx, svisc, BGK_DIAG_CONSISTENT, out) on the
+ *             same state.  No NaN/Inf check (read out and test it).
+ * The reduction partials live in a fixed device array of
+ * 2*8*BGK_NOALLOC_MAX_PARTS + 16 doubles (4 MiB) per GPU, kept until the
+ * process ends; a lattice needing more slots than BGK_NOALLOC_MAX_PARTS is
+ * refused with BGK_EINVAL (one slot per reduction tile of an x-y plane and
+ * z chunk: x-y planes of more than ~2^24 sites in FP32 storage, ~2^23 in
+ * the other kinds, e.g. 4096 x 4096 FP32 fits at any lz).  Every call on one GPU shares that array: calls that
+ * overlap on one GPU (two streams, two host threads) race.                */
+#define BGK_NOALLOC_MAX_PARTS 32768
+int bgk_diagnostics_noalloc(const void *pop, int lx, int ly, int lz, int storage, int flags,
+                            double svisc, double *out, void *stream);
+
 /* Populations of the box [x0,x1) x [y0,y1) x [z0,z1) (0-based, interior
  * sites) into out: 19 planes of (x1-x0)(y1-y0)(z1-z0) doubles, x fastest.
  * Device-side gather replacing the host-array reads of probe_global

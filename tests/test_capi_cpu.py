@@ -54,6 +54,23 @@ def test_create_rejects_invalid_arguments(args, code):
     assert len(L.last_error()) > 0
 
 
+@pytest.mark.parametrize("shape,kind,flags", [
+    ((4224, 4096, 64), L.STORE_F32, 0),   # 33 * 1024 FP32 reduction tiles per plane
+    ((4096, 4096, 8), L.STORE_F64, 0),    # 64 * 1024 FP64 reduction tiles per plane
+    ((0, 8, 8), L.STORE_F32, 0),          # empty lattice
+    ((8, 8, 8), 9, 0),                    # unknown storage kind
+    ((8, 8, 8), L.STORE_F32, L.FLAG_REFCOMPAT_HALOS),  # NOSHIFT is the only flag
+    ((2048, 2048, 1024), L.STORE_F16, 0),  # > 2^31 sites per population
+])
+def test_noalloc_diagnostics_rejects_before_touching_a_gpu(shape, kind, flags):
+    lib = L.load()
+    pop, out = ctypes.c_void_p(256), ctypes.c_void_p(4096)  # never dereferenced
+    assert lib.bgk_diagnostics_noalloc(pop, *shape, kind, flags, 0.01, out, None) == L.EINVAL
+    assert len(L.last_error()) > 0
+    assert lib.bgk_diagnostics_noalloc(None, 8, 8, 8, L.STORE_F32, 0, 0.01, out, None) == \
+        L.EINVAL
+
+
 def test_create_without_gpu_reports_no_device():
     if bgk.device_count() > 0:
         pytest.skip("a GPU is visible")
