@@ -106,21 +106,15 @@ hipError_t timing_end(bgk_ctx *c, int cls, hipStream_t s, hipEvent_t begin) {
 // ------------------------------------------------------------- region boxes
 static Box full_box(const Lattice &L) { return Box{0, L.nx, 0, L.ny, 0, L.nz}; }
 
-// bulk / boundary split of col_MC_masked.  Periodic layout: the reference
-// mask (SRC/init.F90:44-62): bulk = sites at least `border` away from every
-// face.  Slab layout: bulk = planes that do not read a ghost plane.
-static void region_boxes(const bgk_ctx *c, int opt, std::vector<Box> &out) {
+// bulk / border split of col_MC_masked: the reference mask
+// (SRC/init.F90:44-62, tested at SRC/col_MC_masked.F90:114): a site is bulk
+// (mask 0) when it lies at least `border` sites from every face of the
+// rank's box, border (mask 1) otherwise; with 2*border >= a side there is
+// no bulk.  The same on both layouts (a slab's ghost planes are outside the
+// box): with border >= 1 the bulk reads no ghost plane.
+static void mask_boxes(const bgk_ctx *c, int opt, std::vector<Box> &out) {
     const Lattice &L = c->lat;
     out.clear();
-    if (L.layout == BGK_LAYOUT_SLAB_Z) {
-        if (opt == 0) {
-            if (L.nz > 2) out.push_back(Box{0, L.nx, 0, L.ny, 1, L.nz - 1});
-        } else {
-            out.push_back(Box{0, L.nx, 0, L.ny, 0, 1});
-            if (L.nz > 1) out.push_back(Box{0, L.nx, 0, L.ny, L.nz - 1, L.nz});
-        }
-        return;
-    }
     const int b = c->border;
     const bool has_bulk = (2 * b < L.nx) && (2 * b < L.ny) && (2 * b < L.nz);
     if (!has_bulk) {
@@ -138,6 +132,16 @@ static void region_boxes(const bgk_ctx *c, int opt, std::vector<Box> &out) {
     out.push_back(Box{0, L.nx, L.ny - b, L.ny, b, L.nz - b});
     out.push_back(Box{0, b, b, L.ny - b, b, L.nz - b});
     out.push_back(Box{L.nx - b, L.nx, b, L.ny - b, b, L.nz - b});
+}
+
+// the slab's overlapped single step (step_slab): the bulk planes 1..nz-2,
+// which read no ghost plane, and the two boundary planes that do
+static void slab_plane_boxes(const Lattice &L, std::vector<Box> &bulk, std::vector<Box> &bnd) {
+    bulk.clear();
+    bnd.clear();
+    if (L.nz > 2) bulk.push_back(Box{0, L.nx, 0, L.ny, 1, L.nz - 1});
+    bnd.push_back(Box{0, L.nx, 0, L.ny, 0, 1});
+    if (L.nz > 1) bnd.push_back(Box{0, L.nx, 0, L.ny, L.nz - 1, L.nz});
 }
 
 static int collide_boxes(bgk_ctx *c, const std::vector<Box> &boxes, hipStream_t s) {
@@ -178,20 +182,24 @@ static int record_faces(bgk_ctx *c, hipStream_t s) {
 // faces of S_{T-2} and slot[1-cur] those of S_{T-1}.  A single step records
 // S_T into slot[cur] before it; a two-step pass from S_T (copy a) to S_{T+2}
 // (copy 1-a) records S_T into slot[1-a] and writes the faces of its
-// intermediate S_{T+1} into slot[a] from inside the pass.
+// intermediate S_{T+1} into slot[a] from inside the pass; a three-step pass
+// to S_{T+3} writes the faces of S_{T+1} into slot[1-a] and those of S_{T+2}
+// into slot[a] from inside the pass.
 int step_periodic(bgk_ctx *c, int nsteps, bool count) {
     std::vector<Box> all{full_box(c->lat)};
     const bool images = (c->flags & BGK_FLAG_REFCOMPAT_HALOS) != 0;
     const int n0 = nsteps;
-    if (!images && three_step_ok(c->lat)) {
+    if (three_step_ok(c->lat)) {
         // a remainder of one step: two two-step passes rather than a
         // three-step pass and a single step (512^3 FP32: 8.0 against 8.9 ms)
         int n3 = nsteps / 3;
         if (nsteps % 3 == 1 && n3 > 0 && two_step_ok(c->lat)) n3--;
         for (; n3 > 0; n3--, nsteps -= 3) {
+            const int a = c->cur;
             hipEvent_t ev;
             BGK_HIP(timing_begin(c, TCLS_PASS3, c->s0, &ev));
-            BGK_HIP(launch_collide3(c->lat, c->cur, c->omega, c->s0));
+            BGK_HIP(launch_collide3(c->lat, a, c->omega, images ? c->gcache[1 - a] : nullptr,
+                                    images ? c->gcache[a] : nullptr, c->s0));
             BGK_HIP(timing_end(c, TCLS_PASS3, c->s0, ev));
             c->cur ^= 1;
         }
@@ -234,8 +242,7 @@ int step_periodic(bgk_ctx *c, int nsteps, bool count) {
 //   overwrites the buffer s1 boundary(t-1) read.
 static int step_slab(bgk_ctx *c, int nsteps) {
     std::vector<Box> bulk, border;
-    region_boxes(c, 0, bulk);
-    region_boxes(c, 1, border);
+    slab_plane_boxes(c->lat, bulk, border);
     // both streams start after all previously queued work of the context
     BGK_HIP(hipEventRecord(c->ev0, c->s0));
     BGK_HIP(hipStreamWaitEvent(c->s1, c->ev0, 0));
@@ -356,7 +363,7 @@ static int step_slab3(bgk_ctx *c, int ntriples) {
         BGK_HIP(hipStreamWaitEvent(c->s0, c->ev1, 0));  // boundary(k-1) done
         hipEvent_t ev;
         BGK_HIP(timing_begin(c, TCLS_PASS3, c->s0, &ev));
-        BGK_HIP(launch_collide3(L, a, c->omega, c->s0));
+        BGK_HIP(launch_collide3(L, a, c->omega, nullptr, nullptr, c->s0));
         BGK_HIP(timing_end(c, TCLS_PASS3, c->s0, ev));
         // first step
         BGK_HIP(timing_begin(c, TCLS_COLLIDE, c->s1, &ev));
@@ -739,7 +746,7 @@ int bgk_col_MC_masked(bgk_ctx *c, int opt) {
     if (r) return r;
     if (opt != 0 && opt != 1) return fail(BGK_EINVAL, "opt must be 0 (bulk) or 1 (border)");
     std::vector<Box> boxes;
-    region_boxes(c, opt, boxes);
+    mask_boxes(c, opt, boxes);
     r = collide_boxes(c, boxes, c->s0);
     if (r) return r;
     if (opt == 1) {  // SRC/col_MC_masked.F90:369: swap after the border update

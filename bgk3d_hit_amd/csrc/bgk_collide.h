@@ -352,7 +352,8 @@ struct Geo2 {
     long long ystride, zstride;  // element pitches of rows and planes (as Geo)
     unsigned pstride;            // BUF: bytes from population q to q+1
     int xcd_group;               // workgroups per XCD in one round (CUs / 8)
-    void *faces;  // FACES: face images of the first step (k_pack_faces layout)
+    void *faces;   // FACES: face images of the first step (k_pack_faces layout)
+    void *faces2;  // FACES, three-step pass: face images of the second step
 };
 
 static Geo2 make_geo2(const Lattice &L, int tx, int ty, int zc, int *nblocks) {
@@ -373,6 +374,7 @@ static Geo2 make_geo2(const Lattice &L, int tx, int ty, int zc, int *nblocks) {
     g.pstride = (unsigned)((const char *)L.pop[0][1] - (const char *)L.pop[0][0]);
     g.xcd_group = L.ncu >= 8 ? L.ncu / 8 : 1;
     g.faces = nullptr;
+    g.faces2 = nullptr;
     *nblocks = g.tiles_x * g.tiles_y * ((g.zhi - g.zlo + g.zc - 1) / g.zc);
     return g;
 }
@@ -390,6 +392,62 @@ __device__ __forceinline__ void lds_barrier() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
     __builtin_amdgcn_s_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// Face images of the periodic layout (the layout of k_pack_faces): the value
+// s[q] of site (gx, gy, p) goes to the image of each face that population q
+// leaves through, gx[q][gy + ny p], gy[q][gx + nx p], gz[q][gx + nx gy].
+// BUF: through one buffer resource (32-bit offsets: the images hold
+// 19 (ny nz + nx nz + nx ny) values), a vector offset per face and the
+// population's distance as a scalar offset, so that no per-population
+// address stays live in vector registers across the pass's loop.
+template <typename S, bool BUF = false>
+__device__ __forceinline__ void write_faces(void *faces, const Geo2 &g, const S (&s)[19], int gx,
+                                            int gy, int p) {
+    if (gx != 0 && gx != g.nx - 1 && gy != 0 && gy != g.ny - 1 && p != 0 && p != g.nz - 1)
+        return;  // not on a face (most sites)
+    if constexpr (BUF) {
+        // one branch per face (uniform over most waves), then the five
+        // populations that leave through it
+        constexpr uint32_t es = sizeof(S);
+        const __amdgpu_buffer_rsrc_t R = raw_rsrc(faces);
+        const uint32_t nyz = (uint32_t)g.ny * g.nz, nxz = (uint32_t)g.nx * g.nz;
+        const uint32_t oy = 19 * nyz, oz = oy + 19 * nxz;  // elements before gy, gz
+        auto face = [&](int axis, int sgn, uint32_t voff) {
+            asm volatile("" : "+v"(voff));
+#pragma unroll
+            for (int q = 0; q < 18; q++) {
+                const int c = axis == 0 ? cx(q) : (axis == 1 ? cy(q) : cz(q));
+                if (c != sgn) continue;
+                const uint32_t e0 = axis == 0 ? (uint32_t)q * nyz
+                                              : (axis == 1 ? oy + (uint32_t)q * nxz
+                                                           : oz + (uint32_t)q * g.nxy);
+                buf_st<S>(s[q], R, voff, e0 * es);
+            }
+        };
+        const uint32_t vx = ((uint32_t)gy + (uint32_t)g.ny * p) * es;
+        const uint32_t vy = ((uint32_t)gx + (uint32_t)g.nx * p) * es;
+        const uint32_t vz = ((uint32_t)gx + (uint32_t)g.nx * gy) * es;
+        if (gx == g.nx - 1) face(0, 1, vx);
+        if (gx == 0) face(0, -1, vx);
+        if (gy == g.ny - 1) face(1, 1, vy);
+        if (gy == 0) face(1, -1, vy);
+        if (p == g.nz - 1) face(2, 1, vz);
+        if (p == 0) face(2, -1, vz);
+        return;
+    }
+    S *fx = (S *)faces;
+    S *fy = fx + (size_t)19 * g.ny * g.nz;
+    S *fz = fy + (size_t)19 * g.nx * g.nz;
+#pragma unroll
+    for (int q = 0; q < 18; q++) {
+        if (cx(q) != 0 && gx == (cx(q) > 0 ? g.nx - 1 : 0))
+            fx[(size_t)q * g.ny * g.nz + gy + (size_t)g.ny * p] = s[q];
+        if (cy(q) != 0 && gy == (cy(q) > 0 ? g.ny - 1 : 0))
+            fy[(size_t)q * g.nx * g.nz + gx + (size_t)g.nx * p] = s[q];
+        if (cz(q) != 0 && p == (cz(q) > 0 ? g.nz - 1 : 0))
+            fz[(size_t)q * g.nxy + gx + (size_t)g.nx * gy] = s[q];
+    }
 }
 
 // Per-site operations of the two-step pass.  `first` runs at every site of
@@ -581,22 +639,8 @@ __device__ __forceinline__ void two_step_pass(const Pops<S> &src, const Pops<S> 
 #pragma unroll
             for (int q = 0; q < 19; q++)
                 if (pgroup<SGN>(q) == 0) lds[pbase(0) + pslot<SGN>(q)][ey][ex] = s1[q];
-            if (FACES && owner && z >= za && z < zb) {
-                // image of population q on the face it leaves through
-                const int p = wrapz(z);
-                S *fx = (S *)g.faces;
-                S *fy = fx + (size_t)19 * g.ny * g.nz;
-                S *fz = fy + (size_t)19 * g.nx * g.nz;
-#pragma unroll
-                for (int q = 0; q < 18; q++) {
-                    if (cx(q) != 0 && gx == (cx(q) > 0 ? g.nx - 1 : 0))
-                        fx[(size_t)q * g.ny * g.nz + gy + (size_t)g.ny * p] = s1[q];
-                    if (cy(q) != 0 && gy == (cy(q) > 0 ? g.ny - 1 : 0))
-                        fy[(size_t)q * g.nx * g.nz + gx + (size_t)g.nx * p] = s1[q];
-                    if (cz(q) != 0 && p == (cz(q) > 0 ? g.nz - 1 : 0))
-                        fz[(size_t)q * g.nxy + gx + (size_t)g.nx * gy] = s1[q];
-                }
-            }
+            if (FACES && owner && z >= za && z < zb)  // S1 on the faces it leaves through
+                write_faces<S>(g.faces, g, s1, gx, gy, wrapz(z));
         }
         lds_barrier();
         if (second) {  // second step at plane z-1
@@ -741,8 +785,12 @@ inline int three_step_zc(const Lattice &L, int tx, int ty, int zc_max) {
 }
 
 // SLAB: the slab's bulk (z offset g.zoff, no wrap), compiled apart so that
-// the periodic pass carries none of it
-template <typename S, typename C, int TX, int TY, bool SH, bool SLAB = false>
+// the periodic pass carries none of it.  FACES (periodic lattice): also write
+// the face images of the first step's state to g.faces and of the second
+// step's to g.faces2 (the stale halo images the reference's dissipation reads,
+// BGK_FLAG_REFCOMPAT_HALOS); that variant carries one load register set, the
+// face writes' registers would not fit beside the second.
+template <typename S, typename C, int TX, int TY, bool SH, bool SLAB = false, bool FACES = false>
 __global__ __launch_bounds__(collide3_threads(TX, TY), 1) void k_collide3(Pops<S> src,
                                                                           Pops<S> dst, Geo2 g,
                                                                           C omega, C cte1) {
@@ -854,6 +902,8 @@ __global__ __launch_bounds__(collide3_threads(TX, TY), 1) void k_collide3(Pops<S
 #pragma unroll
             for (int q = 0; q < 19; q++)
                 if (pgroup<1>(q) == 0) l2[pbase(0) + pslot<1>(q)][ey - 1][ex - 1] = s2[q];
+            if (FACES && owner && z - 1 >= za && z - 1 < zb)
+                write_faces<S, true>(g.faces2, g, s2, gx, gy, wrapz(z - 1));
         }
         lds_barrier();
         if (active) {
@@ -862,6 +912,10 @@ __global__ __launch_bounds__(collide3_threads(TX, TY), 1) void k_collide3(Pops<S
                 if (pgroup<1>(q) == 1) l1[pbase(1) + pslot<1>(q)][ey][ex] = s1[q];
                 if (pgroup<1>(q) == 2) l1[pb + pslot<1>(q)][ey][ex] = s1[q];
             }
+            // S1 on the faces it leaves through (the collision's temporaries
+            // are dead here)
+            if (FACES && owner && z >= za && z < zb)
+                write_faces<S, true>(g.faces, g, s1, gx, gy, wrapz(z));
         }
         if (owner && it >= 4) {  // S3 at plane z-2
             C f3[19], b[19], ux, uy, uz;
@@ -884,6 +938,7 @@ __global__ __launch_bounds__(collide3_threads(TX, TY), 1) void k_collide3(Pops<S
                 if (pgroup<1>(q) == 1) l2[pbase(1) + pslot<1>(q)][ey - 1][ex - 1] = s2[q];
                 if (pgroup<1>(q) == 2) l2[pb + pslot<1>(q)][ey - 1][ex - 1] = s2[q];
             }
+
             r2O_2 = r2O_1;
             r2O_1 = s2[QO];
             r2R_1 = s2[QR];
@@ -893,14 +948,19 @@ __global__ __launch_bounds__(collide3_threads(TX, TY), 1) void k_collide3(Pops<S
         r1R_1 = s1[QR];
     };
 
-    S ra[19], rb[19];
+    S ra[19];
     load_plane(za - 2, ra);
-    int it = 0;
-    for (; it + 1 < nit; it += 2) {  // two register sets taking turns
-        iteration(it, ra, rb);
-        iteration(it + 1, rb, ra);
+    if constexpr (FACES) {
+        for (int it = 0; it < nit; it++) iteration(it, ra, ra);
+    } else {
+        S rb[19];
+        int it = 0;
+        for (; it + 1 < nit; it += 2) {  // two register sets taking turns
+            iteration(it, ra, rb);
+            iteration(it + 1, rb, ra);
+        }
+        if (it < nit) iteration(it, ra, rb);
     }
-    if (it < nit) iteration(it, ra, rb);
 }
 
 // ------------------------------------ three steps per pass, per-site operation

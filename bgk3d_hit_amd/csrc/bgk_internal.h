@@ -167,11 +167,14 @@ bool two_step_ok(const Lattice &L);
 hipError_t launch_collide2(const Lattice &L, int src, double omega, void *faces,
                            hipStream_t s);
 // three time steps in one pass from copy `src` into copy 1-src (periodic
-// layout; slab: the bulk planes 3 .. nz-4); only when three_step_ok(L)
+// layout; slab: the bulk planes 3 .. nz-4); only when three_step_ok(L).
+// faces1 / faces2 (periodic layout, both or neither): receive the face
+// images of the first / second step (k_pack_faces layout).
 bool three_step_ok(const Lattice &L);
 // thinnest slab the three-step pass takes (its bulk covers planes 3..nz-4)
 constexpr int SLAB3_MIN_NZ = 12;
-hipError_t launch_collide3(const Lattice &L, int src, double omega, hipStream_t s);
+hipError_t launch_collide3(const Lattice &L, int src, double omega, void *faces1, void *faces2,
+                           hipStream_t s);
 // diagno and dissipation in one pass: partials holds 2 * 8 * parts doubles,
 // out 16 (out[0..7] as launch_diagno's, out[8..15] as launch_dissipation's)
 hipError_t launch_diag_diss(const Lattice &L, int cur, const void *gcache, double svisc,

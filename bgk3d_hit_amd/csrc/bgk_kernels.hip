@@ -78,14 +78,22 @@ static hipError_t collide2_t(const Lattice &L, int src, double omega, void *face
 // three steps from copy `src` into copy 1-src (periodic lattice, or the
 // bulk planes 3 .. nz-4 of a slab; 32-bit offsets)
 template <typename S, typename C, bool SH>
-static hipError_t collide3_sh(const Lattice &L, int src, double omega, hipStream_t s) {
+static hipError_t collide3_sh(const Lattice &L, int src, double omega, void *faces1,
+                              void *faces2, hipStream_t s) {
     using T = Tile3<S, C>;
     const int zc = three_step_zc(L, T::X, T::Y, T::ZC);
     if (zc <= 0) return hipErrorInvalidValue;  // caller checks three_step_ok first
     int nblocks;
     Geo2 g = make_geo2(L, T::X, T::Y, zc, &nblocks);
     const C w = (C)omega, cte1 = (C)(L.shifted ? 1.0 : 0.0);
-    if (g.periodic_z) {
+    if (faces1 || faces2) {
+        if (!g.periodic_z || !faces1 || !faces2) return hipErrorInvalidValue;
+        g.faces = faces1;
+        g.faces2 = faces2;
+        hipLaunchKernelGGL((k_collide3<S, C, T::X, T::Y, SH, false, true>), dim3(nblocks),
+                           dim3(collide3_threads(T::X, T::Y)), 0, s, pops_of<S>(L, src),
+                           pops_of<S>(L, 1 - src), g, w, cte1);
+    } else if (g.periodic_z) {
         hipLaunchKernelGGL((k_collide3<S, C, T::X, T::Y, SH>), dim3(nblocks),
                            dim3(collide3_threads(T::X, T::Y)), 0, s, pops_of<S>(L, src),
                            pops_of<S>(L, 1 - src), g, w, cte1);
@@ -100,9 +108,10 @@ static hipError_t collide3_sh(const Lattice &L, int src, double omega, hipStream
     return hipGetLastError();
 }
 template <typename S, typename C>
-static hipError_t collide3_t(const Lattice &L, int src, double omega, hipStream_t s) {
-    return L.shifted ? collide3_sh<S, C, true>(L, src, omega, s)
-                     : collide3_sh<S, C, false>(L, src, omega, s);
+static hipError_t collide3_t(const Lattice &L, int src, double omega, void *faces1,
+                             void *faces2, hipStream_t s) {
+    return L.shifted ? collide3_sh<S, C, true>(L, src, omega, faces1, faces2, s)
+                     : collide3_sh<S, C, false>(L, src, omega, faces1, faces2, s);
 }
 
 // --------------------------------------------------------------- reductions
@@ -739,8 +748,9 @@ bool three_step_ok(const Lattice &L) {
     default: return false;
     }
 }
-hipError_t launch_collide3(const Lattice &L, int src, double omega, hipStream_t s) {
-    BGK_DISPATCH(L.storage, collide3_t, L, src, omega, s);
+hipError_t launch_collide3(const Lattice &L, int src, double omega, void *faces1, void *faces2,
+                           hipStream_t s) {
+    BGK_DISPATCH(L.storage, collide3_t, L, src, omega, faces1, faces2, s);
 }
 hipError_t launch_collide2(const Lattice &L, int src, double omega, void *faces,
                            hipStream_t s) {

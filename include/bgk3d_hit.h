@@ -133,15 +133,16 @@ int bgk_col_MC(bgk_ctx *ctx);
 
 /* collide restricted to the bulk (opt = 0) or the boundary region (opt = 1),
  * swap only after opt = 1: replaces col_MC_masked (SRC/col_MC_masked.F90:34-451)
- * without a mask array.  Slab layout: the boundary region is the two planes
- * that read ghosts; periodic layout: sites within `border` of a face.      */
+ * without a mask array.  The regions are the reference mask's
+ * (SRC/init.F90:44-62) on both layouts: border = sites within `border`
+ * (bgk_set_border) of a face of the box, bulk = the rest (none when
+ * 2*border >= a side).  On the slab, call bgk_boundaries before opt = 1.  */
 int bgk_col_MC_masked(bgk_ctx *ctx, int opt);
 
 /* nsteps of the time loop body boundaries + col_MC (SRC/bgk3d_HIT.F90:127-146
  * without diagnostic).  Where the lattice is large enough (unless
  * BGK_FLAG_SINGLE_STEP): three steps per pass over the populations (not with
- * BGK_FLAG_REFCOMPAT_HALOS or BGK_FLAG_NO_THREE_STEP; slabs of 12 planes and
- * more), two steps per pass for a remainder of two (two such passes for a
+ * BGK_FLAG_NO_THREE_STEP; slabs of 12 planes and more), two steps per pass for a remainder of two (two such passes for a
  * remainder of one), single steps otherwise; bit-identical to nsteps calls
  * of bgk_col_MC, stale halo images and slab ghost planes included.  Slab
  * layout: the interior collide overlaps the halo pack/exchange/unpack and

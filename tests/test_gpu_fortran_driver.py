@@ -158,16 +158,17 @@ def test_driver_two_step_pairs_match_single_steps(gpu, tmp_path):
 
 
 @pytest.mark.gpu
-def test_driver_refcompat_two_step_pairs(gpu, tmp_path):
+def test_driver_refcompat_fused_passes(gpu, tmp_path):
     """The reference-compatible diagnostics (stale halo images) with the
-    driver's step pairs on a 256^3 lattice and an odd icheck: equal to the
-    reference call order (boundaries + col_MC every step) exactly."""
+    driver's fused passes (7 steps between outputs: a three-step pass and two
+    two-step passes) on a 256^3 lattice: equal to the reference call order
+    (boundaries + col_MC every step) exactly."""
     n, omega, itfin, icheck = 256, 1.95, 21, 7
     svisc = (2.0 / omega - 1.0) / 6.0
     deck = (f"&parameters\nlx={n}\nly={n}\nlz={n}\nomega={omega}\nitfin={itfin}\n"
             f"icheck={icheck}\ninit='hit'\nk0=4.0\nurms=0.02\nseed=99\n"
             f"storage='f32'\ndiag='refcompat'\n/\n")
-    print("[gpu-case] fortran driver 256^3 f32 refcompat pairs", flush=True)
+    print("[gpu-case] fortran driver 256^3 f32 refcompat fused passes", flush=True)
     run_driver(tmp_path, deck)
     dg = np.loadtxt(tmp_path / "diagno.full.dat", ndmin=2)
     ds = np.loadtxt(tmp_path / "dissipation.full.dat", ndmin=2)

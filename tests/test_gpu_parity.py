@@ -187,6 +187,7 @@ def test_slab_masked_path(gpu):
     u = bgk.hit_velocity(*shape, 2.0, 0.03, 9)
     p = bgk.Lattice(*shape, omega=1.9, storage="f32")
     s = bgk.Lattice(*shape, omega=1.9, storage="f32", layout="slab")
+    s.set_border(2)
     p.init_equilibrium(u)
     s.init_equilibrium(u)
     for _ in range(5):
@@ -195,6 +196,47 @@ def test_slab_masked_path(gpu):
         s.col_MC_masked(0)
         s.col_MC_masked(1)
     assert np.array_equal(s.get_populations(), p.get_populations())
+
+
+@pytest.mark.parametrize("kind", ["f64", "f32", "f32mix", "f16", "bf16"])
+@pytest.mark.parametrize("layout", ["periodic", "slab"])
+@pytest.mark.parametrize("shape,border", [((23, 17, 13), 3), ((20, 14, 11), 1),
+                                          ((16, 12, 9), 4), ((12, 10, 9), 5)])
+def test_masked_regions_against_oracle(gpu, kind, layout, shape, border):
+    """col_MC_masked against the oracle's restatement of the reference mask
+    (SRC/init.F90:44-62; SRC/col_MC_masked.F90:114,355-369).  The bulk call
+    (opt 0) runs at one omega and the border call (opt 1) at another, so
+    every site's result shows which call updated it: a swapped, missing or
+    doubled region changes the bits.  (12, 10, 9) with border 5 has no bulk."""
+    say("masked vs oracle", kind, layout, shape, border)
+    w_bulk, w_border = 1.93, 1.71
+    g, o, _ = hit_pair(shape, 2.0, 0.03, 31, w_bulk, kind)
+    if layout == "slab":
+        g.close()
+        g = bgk.Lattice(*shape, omega=w_bulk, storage=kind, layout="slab")
+        g.init_equilibrium(bgk.hit_velocity(*shape, 2.0, 0.03, 31))
+    g.set_border(border)
+    for _ in range(3):
+        if layout == "slab":
+            g.boundaries()
+        g.set_omega(w_bulk)
+        g.col_MC_masked(0)
+        g.set_omega(w_border)
+        g.col_MC_masked(1)
+        o.bcond_step3()
+        o.set_omega(w_bulk)
+        o.col_MC_masked(0, border)
+        o.set_omega(w_border)
+        o.col_MC_masked(1, border)
+    fg, fo = g.get_populations(), o.get_populations()
+    assert np.array_equal(fg, fo), np.abs(fg - fo).max()
+    # the two omegas must leave different results (the test would pin
+    # nothing otherwise): a full step at the border omega differs
+    if 2 * border < min(shape):
+        f_all = bgk.Lattice(*shape, omega=w_border, storage=kind)
+        f_all.init_equilibrium(bgk.hit_velocity(*shape, 2.0, 0.03, 31))
+        f_all.step(3)
+        assert not np.array_equal(f_all.get_populations(), fg)
 
 
 def test_device_hit_equals_host_hit(gpu):
@@ -223,21 +265,24 @@ def test_nonfinite_populations_are_reported(gpu):
         g.dissipation(0.01)
 
 
-@pytest.mark.parametrize("kind", ["f64", "f32", "f16"])
-def test_refcompat_images_with_two_step_passes(gpu, kind):
-    """With the reference's stale halo images the two-step pass records the
-    faces of its source state and writes those of its intermediate state:
-    the refcompat dissipation after any mix of passes and single steps equals
+@pytest.mark.parametrize("kind", ["f64", "f32", "f32mix", "f16", "bf16"])
+@pytest.mark.parametrize("three_step", [True, False])
+@pytest.mark.parametrize("shape", [(70, 9, 20), (33, 21, 7), (130, 24, 36)])
+def test_refcompat_images_with_fused_passes(gpu, kind, three_step, shape):
+    """With the reference's stale halo images a two-step pass records the
+    faces of its source state and writes those of its intermediate state; a
+    three-step pass writes the faces of both of its intermediate states: the
+    refcompat dissipation after any mix of passes and single steps equals
     the single-step path's exactly."""
-    shape = (70, 9, 20)
-    say("refcompat two-step", kind)
+    say("refcompat fused", kind, three_step, shape)
     u = bgk.hit_velocity(*shape, 2.0, 0.03, 17)
-    a = bgk.Lattice(*shape, omega=1.93, storage=kind, refcompat_halos=True, two_step="always")
+    a = bgk.Lattice(*shape, omega=1.93, storage=kind, refcompat_halos=True, two_step="always",
+                    three_step=three_step, timing=True)
     b = bgk.Lattice(*shape, omega=1.93, storage=kind, refcompat_halos=True, two_step=False)
     a.init_equilibrium(u)
     b.init_equilibrium(u)
     nu = bgk.svisc_from_omega(1.93)
-    for n in (2, 3, 1, 2, 4, 5, 0, 2):
+    for n in (2, 3, 1, 2, 4, 5, 0, 6, 2, 9):
         if n == 0:  # the reference call order, one step
             for x in (a, b):
                 x.boundaries()
@@ -248,6 +293,11 @@ def test_refcompat_images_with_two_step_passes(gpu, kind):
         assert np.array_equal(a.get_populations(), b.get_populations()), n
         assert np.array_equal(a.dissipation(nu, refcompat=True),
                               b.dissipation(nu, refcompat=True)), n
+        assert np.array_equal(a.diagnostics(nu, refcompat=True)[1],
+                              b.dissipation(nu, refcompat=True)), n
+    depth = a.timing_passes()
+    assert (depth[3][1] > 0) == three_step, depth
+    assert depth[2][1] > 0, depth
 
 
 def test_refcompat_dissipation_needs_flag(gpu):

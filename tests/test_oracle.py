@@ -240,6 +240,34 @@ def test_diagno_uniform_state():
     assert np.allclose(d, [1, 0.01, 0.02, -0.03, 0.5 * 0.0014], atol=1e-15)
 
 
+def test_diagno_accumulation_in_the_compute_kind():
+    """acc_mykind: the reference's FP32 build sums diagno's per-site moments
+    serially in single precision (SRC/diagno.F90:107-111).  On a uniform
+    state every site contributes the same value v of each moment; the FP64
+    mean of N identical floats is v itself (N v exact), and the FP32 sums
+    must be the serial float32 sums of N copies of v, which numpy's
+    sequential cumsum computes.  At N = 2^18 the float32 rounding is
+    visible against the FP64 sums."""
+    shape = (64, 64, 64)
+    N = 64 ** 3
+    u = np.zeros((3, 64, 64, 64))
+    u[0], u[1], u[2] = 0.031, -0.017, 0.0093
+    o = oracle.OracleLattice(*shape, omega=1.9, kind="f32")
+    o.init_equilibrium(u)
+    d64, d32 = o.diagno(), o.diagno(acc_mykind=True)
+    serial = {}
+    for k in range(5):
+        v = np.float32(d64[k])
+        assert float(v) == d64[k]
+        serial[k] = np.cumsum(np.full(N, v, dtype=np.float32), dtype=np.float32)[-1]
+        assert d32[k] == ((float(serial[k]) / 64) / 64) / 64, k
+    assert np.abs(d32 - d64).max() > 1e-6 * np.abs(d64).max()
+    # with the ztot bug the FP32 running sum ends as ytot + zj(l,m,n) in float32
+    bug = o.diagno(ztot_bug=True, acc_mykind=True)
+    zt = np.float32(serial[2] + np.float32(d64[3]))
+    assert bug[3] == ((float(zt) / 64) / 64) / 64
+
+
 def test_dissipation_zero_on_uniform_equilibrium():
     shape = (4, 5, 6)
     u = np.zeros((3, 6, 5, 4))
