@@ -59,6 +59,7 @@ SIGNATURES = {
     "bgk_synchronize": [_p],
     "bgk_timing": [_p, _dp, _c_int],
     "bgk_timing_passes": [_p, _dp, _c_int],
+    "bgk_timing_adjoint": [_p, _dp, _c_int],
     "bgk_adjoint_ke": [_p, _dp, _c_int, _c_int, _c_int, _dp, _dp, _dp],
     "bgk_adjoint_ke_device": [_p, _p, _c_int, _c_int, _c_int, _dp, _p, _dp],
     "bgk_forward_ke": [_p, _dp, _c_int, _dp],

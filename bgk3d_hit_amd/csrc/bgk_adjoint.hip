@@ -265,12 +265,31 @@ struct Adj {
     bgk_ctx *c;
     size_t state_bytes; // one copy of the populations (all 19, interleaved)
     size_t csize;       // sizeof compute kind
+    size_t es;          // sizeof storage kind
+    void *own[2];       // the lattice's own two copies
     std::vector<void *> allocs;
     size_t bytes = 0;
     long long recomputed = 0;
 
+    Adj(bgk_ctx *ctx, size_t storage_bytes) : c(ctx), es(storage_bytes) {
+        own[0] = c->lat.pop[0][0];
+        own[1] = c->lat.pop[1][0];
+        state_bytes = ((size_t)c->lat.copy_elems * es + 255) & ~(size_t)255;
+    }
     ~Adj() {
+        // the lattice returns to its own copies (whose contents are whatever
+        // the adjoint left there) before the checkpoint buffers are freed
+        // (hipFree waits for the device)
+        set_copy(0, own[0]);
+        set_copy(1, own[1]);
         for (void *p : allocs) (void)hipFree(p);
+    }
+    // point copy k of the lattice at the block `base` (the lattice's
+    // interleaved layout): checkpoints change places with the lattice's
+    // copies instead of being copied
+    void set_copy(int k, const void *base) {
+        Lattice &L = c->lat;
+        for (int q = 0; q < 19; q++) L.pop[k][q] = (char *)base + (size_t)q * L.nx * es;
     }
     int alloc(void **p, size_t n, const char *what) {
         hipError_t e = hipMalloc(p, n);
@@ -280,33 +299,31 @@ struct Adj {
         return BGK_OK;
     }
     // copy the current state (copy c->cur, whose block starts at population
-    // 0's pointer) to / from a slot
-    int save(void *slot) {
+    // 0's pointer) to / from a slot, on stream s
+    int save(void *slot, hipStream_t s) {
         const Lattice &L = c->lat;
-        BGK_HIP(hipMemcpyAsync(slot, L.pop[c->cur][0], state_bytes, hipMemcpyDeviceToDevice,
-                               c->s0));
+        BGK_HIP(hipMemcpyAsync(slot, L.pop[c->cur][0], state_bytes, hipMemcpyDeviceToDevice, s));
         return BGK_OK;
     }
-    int restore(const void *slot) {
+    int restore(const void *slot, hipStream_t s) {
         const Lattice &L = c->lat;
-        BGK_HIP(hipMemcpyAsync(L.pop[c->cur][0], slot, state_bytes, hipMemcpyDeviceToDevice,
-                               c->s0));
+        BGK_HIP(hipMemcpyAsync(L.pop[c->cur][0], slot, state_bytes, hipMemcpyDeviceToDevice, s));
         return BGK_OK;
     }
 };
 
 template <typename S, typename C>
-hipError_t launch_tape_step(bgk_ctx *c, C *tape) {
+hipError_t launch_tape_step(bgk_ctx *c, C *tape, hipStream_t s) {
     const Lattice &L = c->lat;
     int ntiles;
     Geo g = make_geo<S>(L, Box{0, L.nx, 0, L.ny, 0, L.nz}, &ntiles);
     if (offsets_fit_32(L, sizeof(S)))
         hipLaunchKernelGGL((k_collide<S, C, true, true>), dim3(ntiles),
-                           dim3(Tile<S>::X, Tile<S>::Y), 0, c->s0, pops_of<S>(L, c->cur),
+                           dim3(Tile<S>::X, Tile<S>::Y), 0, s, pops_of<S>(L, c->cur),
                            pops_of<S>(L, 1 - c->cur), g, (C)c->omega, (C)1, tape, L.nsite);
     else
         hipLaunchKernelGGL((k_collide<S, C, false, true>), dim3(ntiles),
-                           dim3(Tile<S>::X, Tile<S>::Y), 0, c->s0, pops_of<S>(L, c->cur),
+                           dim3(Tile<S>::X, Tile<S>::Y), 0, s, pops_of<S>(L, c->cur),
                            pops_of<S>(L, 1 - c->cur), g, (C)c->omega, (C)1, tape, L.nsite);
     c->cur ^= 1;
     return hipGetLastError();
@@ -314,14 +331,14 @@ hipError_t launch_tape_step(bgk_ctx *c, C *tape) {
 
 // two taped forward steps (slices tape0, tape1) in one pass
 template <typename S, typename C, bool O32>
-hipError_t launch_tape2_o(bgk_ctx *c, C *tape0, C *tape1) {
+hipError_t launch_tape2_o(bgk_ctx *c, C *tape0, C *tape1, hipStream_t s) {
     const Lattice &L = c->lat;
     using T = Tile2<S, C, O32>;
     int nblocks;
     const Geo2 g = make_geo2(L, T::X, T::Y, two_step_zc(L, T::X, T::Y, T::ZC), &nblocks);
     const CollideOp<C, true, true> op{(C)c->omega, (C)1, tape0, tape1, (size_t)L.nsite};
     hipLaunchKernelGGL((k_collide2<S, C, O32, T::X, T::Y, T::MINB, true, false, true>), dim3(nblocks),
-                       dim3(collide2_threads(T::X, T::Y)), 0, c->s0, pops_of<S>(L, c->cur),
+                       dim3(collide2_threads(T::X, T::Y)), 0, s, pops_of<S>(L, c->cur),
                        pops_of<S>(L, 1 - c->cur), g, op);
     c->cur ^= 1;
     return hipGetLastError();
@@ -329,14 +346,14 @@ hipError_t launch_tape2_o(bgk_ctx *c, C *tape0, C *tape1) {
 
 // three taped forward steps (slices tape0..tape2) in one pass
 template <typename S, typename C>
-hipError_t launch_tape3(bgk_ctx *c, C *tape0, C *tape1, C *tape2) {
+hipError_t launch_tape3(bgk_ctx *c, C *tape0, C *tape1, C *tape2, hipStream_t s) {
     const Lattice &L = c->lat;
     using T = Tile3<S, C>;
     int nblocks;
     const Geo2 g = make_geo2(L, T::X, T::Y, three_step_zc(L, T::X, T::Y, T::ZC), &nblocks);
     const CollideOp3<C, true, true> op{(C)c->omega, (C)1, tape0, tape1, tape2, (size_t)L.nsite};
     hipLaunchKernelGGL((k_tape3<S, C, T::X, T::Y>), dim3(nblocks),
-                       dim3(collide3_threads(T::X, T::Y)), 0, c->s0, pops_of<S>(L, c->cur),
+                       dim3(collide3_threads(T::X, T::Y)), 0, s, pops_of<S>(L, c->cur),
                        pops_of<S>(L, 1 - c->cur), g, op);
     c->cur ^= 1;
     return hipGetLastError();
@@ -361,23 +378,36 @@ int adjoint_triples(const Lattice &L, int n) {
     return n3;
 }
 
-// taped forward steps: slice k of `tape` records step k of the n steps
+// taped forward steps on stream s: slice k of `tape` records step k of the
+// n steps
 template <typename S, typename C>
-int run_taped(bgk_ctx *c, C *tape, int n) {
+int run_taped(bgk_ctx *c, C *tape, int n, hipStream_t s) {
     const Lattice &L = c->lat;
     const size_t slice = (size_t)3 * L.nsite;
+    hipEvent_t ev;
     int k = 0;
-    for (int n3 = adjoint_triples<C>(L, n); n3 > 0; n3--, k += 3)
+    for (int n3 = adjoint_triples<C>(L, n); n3 > 0; n3--, k += 3) {
+        BGK_HIP(timing_begin(c, TCLS_TAPE, s, &ev));
         BGK_HIP((launch_tape3<S, C>(c, tape + k * slice, tape + (k + 1) * slice,
-                                    tape + (k + 2) * slice)));
+                                    tape + (k + 2) * slice, s)));
+        BGK_HIP(timing_end(c, TCLS_TAPE, s, ev));
+    }
     if (two_step_ok(L)) {
         const bool o32 = offsets_fit_32(L, sizeof(S));
-        for (; k + 1 < n; k += 2)
-            BGK_HIP(o32 ? (launch_tape2_o<S, C, true>(c, tape + k * slice, tape + (k + 1) * slice))
+        for (; k + 1 < n; k += 2) {
+            BGK_HIP(timing_begin(c, TCLS_TAPE, s, &ev));
+            BGK_HIP(o32 ? (launch_tape2_o<S, C, true>(c, tape + k * slice,
+                                                      tape + (k + 1) * slice, s))
                         : (launch_tape2_o<S, C, false>(c, tape + k * slice,
-                                                       tape + (k + 1) * slice)));
+                                                       tape + (k + 1) * slice, s)));
+            BGK_HIP(timing_end(c, TCLS_TAPE, s, ev));
+        }
     }
-    for (; k < n; k++) BGK_HIP((launch_tape_step<S, C>(c, tape + k * slice)));
+    for (; k < n; k++) {
+        BGK_HIP(timing_begin(c, TCLS_TAPE, s, &ev));
+        BGK_HIP((launch_tape_step<S, C>(c, tape + k * slice, s)));
+        BGK_HIP(timing_end(c, TCLS_TAPE, s, ev));
+    }
     return BGK_OK;
 }
 
@@ -430,24 +460,31 @@ template <typename C>
 int adjoint_sweep(bgk_ctx *c, C *&lin, C *&lout, const C *tape, int n) {
     const Lattice &L = c->lat;
     const size_t slice = (size_t)3 * L.nsite;
+    hipEvent_t ev;
     int k = n - 1;
     for (int n3 = adjoint_triples<C>(L, n); n3 > 0; n3--, k -= 3) {
+        BGK_HIP(timing_begin(c, TCLS_ADJOINT, c->s0, &ev));
         BGK_HIP(launch_adjoint3<C>(c, lin, lout, tape + k * slice, tape + (k - 1) * slice,
                                    tape + (k - 2) * slice));
+        BGK_HIP(timing_end(c, TCLS_ADJOINT, c->s0, ev));
         std::swap(lin, lout);
     }
     if (two_step_ok(L)) {
         const bool o32 = offsets_fit_32(L, sizeof(C));
         for (; k >= 1; k -= 2) {
+            BGK_HIP(timing_begin(c, TCLS_ADJOINT, c->s0, &ev));
             BGK_HIP(o32 ? (launch_adjoint2_o<C, true>(c, lin, lout, tape + k * slice,
                                                       tape + (k - 1) * slice))
                         : (launch_adjoint2_o<C, false>(c, lin, lout, tape + k * slice,
                                                        tape + (k - 1) * slice)));
+            BGK_HIP(timing_end(c, TCLS_ADJOINT, c->s0, ev));
             std::swap(lin, lout);
         }
     }
     for (; k >= 0; k--) {
+        BGK_HIP(timing_begin(c, TCLS_ADJOINT, c->s0, &ev));
         BGK_HIP(launch_adjoint_step<C>(c, lin, lout, tape + k * slice));
+        BGK_HIP(timing_end(c, TCLS_ADJOINT, c->s0, ev));
         std::swap(lin, lout);
     }
     return BGK_OK;
@@ -462,19 +499,50 @@ hipError_t launch_adjoint_step(bgk_ctx *c, const C *lin, C *lout, const C *tape)
     return hipGetLastError();
 }
 
+// time steps of the first pass step_periodic (first_pass_steps) and
+// run_taped (taped_first_pass) take for n steps; they take the same passes
+// for the rest of them afterwards
+int first_pass_steps(const Lattice &L, int n) {
+    if (n <= 0) return 0;
+    if (three_step_ok(L)) {
+        int n3 = n / 3;
+        if (n % 3 == 1 && n3 > 0 && two_step_ok(L)) n3--;
+        if (n3 > 0) return 3;
+    }
+    return (two_step_ok(L) && n >= 2) ? 2 : 1;
+}
+
+template <typename C>
+int taped_first_pass(const Lattice &L, int n) {
+    if (n <= 0) return 0;
+    if (adjoint_triples<C>(L, n) > 0) return 3;
+    return (two_step_ok(L) && n >= 2) ? 2 : 1;
+}
+
 double ms_since(std::chrono::steady_clock::time_point t0) {
     return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0)
         .count();
 }
+
+// two events per role, used alternately by consecutive segments
+struct EventPair {
+    hipEvent_t e[2] = {nullptr, nullptr};
+    ~EventPair() {
+        for (hipEvent_t x : e)
+            if (x) (void)hipEventDestroy(x);
+    }
+    int create() {
+        for (hipEvent_t &x : e) BGK_HIP(hipEventCreateWithFlags(&x, hipEventDisableTiming));
+        return BGK_OK;
+    }
+};
 
 template <typename S, typename C>
 int adjoint_t(bgk_ctx *c, const double *u0, int T, int K, int levels, double *J, double *grad,
               double stats[4]) {
     const Lattice &L = c->lat;
     const long long N = L.nsite;
-    Adj a;
-    a.c = c;
-    a.state_bytes = ((size_t)L.copy_elems * sizeof(S) + 255) & ~(size_t)255;
+    Adj a(c, sizeof(S));
     a.csize = sizeof(C);
     if (K < 1) K = 1;
     if (levels == 0) K = std::max(T, 1);
@@ -489,27 +557,47 @@ int adjoint_t(bgk_ctx *c, const double *u0, int T, int K, int levels, double *J,
         if ((r = a.alloc(&coarse[i], a.state_bytes, "hipMalloc(checkpoint)"))) return r;
     for (int i = 0; i < (int)fine.size(); i++)
         if ((r = a.alloc(&fine[i], a.state_bytes, "hipMalloc(fine checkpoint)"))) return r;
-    C *tape = nullptr, *lamA = nullptr, *lamB = nullptr;
-    if ((r = a.alloc((void **)&tape, sizeof(C) * 3 * (size_t)N * tape_steps, "hipMalloc(tape)")))
-        return r;
+    // two tape buffers (one segment each) when segments are re-taped, so that
+    // the re-tape of the next segment overlaps the sweep of this one
+    C *tape[2] = {nullptr, nullptr}, *lamA = nullptr, *lamB = nullptr;
+    const size_t tape_bytes = sizeof(C) * 3 * (size_t)N * tape_steps;
+    for (int b = 0; b < (nseg > 1 ? 2 : 1); b++)
+        if ((r = a.alloc((void **)&tape[b], tape_bytes, "hipMalloc(tape)"))) return r;
     if ((r = a.alloc((void **)&lamA, sizeof(C) * 19 * (size_t)N, "hipMalloc(lambda)"))) return r;
     if ((r = a.alloc((void **)&lamB, sizeof(C) * 19 * (size_t)N, "hipMalloc(lambda)"))) return r;
+    EventPair ev_tape, ev_sweep;
+    if ((r = ev_tape.create()) || (r = ev_sweep.create())) return r;
+    hipStream_t s1 = c->s1;
 
-    // ---- forward: f^0 from u0, coarse checkpoints every K2 steps; the last
-    // segment's tape is recorded on the way
+    // ---- forward (stream s0): f^0 from u0, coarse checkpoints every K2
+    // steps; the last segment's tape is recorded on the way
     const int last_s0 = nseg > 0 ? (nseg - 1) * K : 0;
     auto t0 = std::chrono::steady_clock::now();
     BGK_HIP(launch_init_equilibrium(L, c->cur, u0, N, 0, N, c->s0));
+    // Checkpoint i is kept in place: the buffer holding S_{i K2} becomes
+    // checkpoint i once the next pass has read it, and the lattice carries on
+    // in a spare buffer (the checkpoint allocations), so no state is copied.
+    std::vector<void *> spare(coarse.rbegin(), coarse.rend()), ck(ncoarse, nullptr);
     for (int t = 0; t < T;) {
-        if (t % K2 == 0 && (r = a.save(coarse[t / K2]))) return r;
-        // up to the next checkpoint; the last segment taped (two-step passes
+        // up to the next checkpoint; the last segment taped (fused passes
         // where they pay)
         const int n = std::min(K2 - t % K2, (t >= last_s0 ? T : last_s0) - t);
-        if (t >= last_s0)
-            r = run_taped<S, C>(c, tape + (size_t)(t - last_s0) * 3 * N, n);
-        else
-            r = step_periodic(c, n, false);
-        if (r) return r;
+        const bool taped = t >= last_s0;
+        C *tp = taped ? tape[0] + (size_t)(t - last_s0) * 3 * N : nullptr;
+        auto run = [&](int m) {
+            return taped ? run_taped<S, C>(c, tp, m, c->s0) : step_periodic(c, m, false, c->s0);
+        };
+        int done = 0;
+        if (t % K2 == 0) {
+            const int k = c->cur;
+            ck[t / K2] = c->lat.pop[k][0];
+            done = taped ? taped_first_pass<C>(L, n) : first_pass_steps(L, n);
+            if ((r = run(done))) return r;
+            a.set_copy(k, spare.back());
+            spare.pop_back();
+            if (tp) tp += (size_t)done * 3 * N;
+        }
+        if (n > done && (r = run(n - done))) return r;
         t += n;
     }
     if ((r = reduce_ke(c, c->cur, J))) return r;  // synchronises
@@ -519,45 +607,69 @@ int adjoint_t(bgk_ctx *c, const double *u0, int T, int K, int levels, double *J,
     BGK_HIP(hipStreamSynchronize(c->s0));
     const double fwd_ms = ms_since(t0);
 
-    // ---- backward, one coarse segment at a time
+    // ---- backward.  The segments in backward order; segment 0 (the last)
+    // was taped by the forward into tape[0], segment k uses tape[k % 2].
+    // The sweeps run on s0; everything on the lattice state (fine checkpoints
+    // of a coarse segment, restores, re-taping) runs on s1, so the re-tape of
+    // segment k+1 overlaps the sweep of segment k.  Cross-stream order:
+    //   sweep k waits for the re-tape of segment k (ev_tape);
+    //   the re-tape of segment k+1 into tape[(k+1) % 2] waits for the sweep
+    //   of segment k-1, which read that buffer (ev_sweep).
     t0 = std::chrono::steady_clock::now();
-    C *lin = lamA, *lout = lamB;
-    bool tape_is_last = true;  // the tape holds the last segment
+    struct Seg {
+        int cs, j, s0, s1;
+    };
+    std::vector<Seg> segs;
     for (int cs = ncoarse - 1; cs >= 0; cs--) {
         const int c0 = cs * K2, c1 = std::min(T, c0 + K2);
         const int nf = (c1 - c0 + K - 1) / K;
-        // fine checkpoints are needed for every segment still to be taped
-        const int need = tape_is_last ? nf - 1 : nf;
-        if (need > 0 && M > 1) {
-            if ((r = a.restore(coarse[cs]))) return r;
-            {
-                for (int j = 0; j < need; j++) {
-                    if ((r = a.save(fine[j]))) return r;
-                    if (j == need - 1) break;
-                    if ((r = step_periodic(c, K, false))) return r;
-                    a.recomputed += K;
-                }
+        for (int j = nf - 1; j >= 0; j--)
+            segs.push_back(Seg{cs, j, c0 + j * K, std::min(c1, c0 + j * K + K)});
+    }
+    int prepared_cs = ncoarse;  // coarse segment whose fine checkpoints are in `fine`
+    auto retape = [&](int k) -> int {
+        const Seg &g = segs[k];
+        int rr;
+        if (k >= 2) BGK_HIP(hipStreamWaitEvent(s1, ev_sweep.e[k % 2], 0));
+        if (M > 1 && g.cs != prepared_cs) {
+            // fine checkpoints 0 .. g.j of this coarse segment (g.j is its
+            // first segment to re-tape)
+            if ((rr = a.restore(ck[g.cs], s1))) return rr;
+            for (int jj = 0; jj <= g.j; jj++) {
+                if ((rr = a.save(fine[jj], s1))) return rr;
+                if (jj == g.j) break;
+                if ((rr = step_periodic(c, K, false, s1))) return rr;
+                a.recomputed += K;
             }
+            prepared_cs = g.cs;
         }
-        for (int j = nf - 1; j >= 0; j--) {
-            const int s0 = c0 + j * K, s1 = std::min(c1, s0 + K);
-            if (tape_is_last) {
-                tape_is_last = false;
-            } else {
-                if (M > 1) {
-                    if ((r = a.restore(fine[j]))) return r;
-                } else if ((r = a.restore(coarse[cs]))) {
-                    return r;
-                }
-                if ((r = run_taped<S, C>(c, tape, s1 - s0))) return r;
-                a.recomputed += s1 - s0;
-            }
-            if ((r = adjoint_sweep<C>(c, lin, lout, tape, s1 - s0))) return r;
-        }
+        // the first taped pass reads the checkpoint in place
+        const int n = g.s1 - g.s0, kk = c->cur, first = taped_first_pass<C>(L, n);
+        void *mine = c->lat.pop[kk][0];
+        a.set_copy(kk, M > 1 ? fine[g.j] : ck[g.cs]);
+        if ((rr = run_taped<S, C>(c, tape[k % 2], first, s1))) return rr;
+        a.set_copy(kk, mine);
+        if (n > first &&
+            (rr = run_taped<S, C>(c, tape[k % 2] + (size_t)first * 3 * N, n - first, s1)))
+            return rr;
+        a.recomputed += g.s1 - g.s0;
+        BGK_HIP(hipEventRecord(ev_tape.e[k % 2], s1));
+        return BGK_OK;
+    };
+    C *lin = lamA, *lout = lamB;
+    const int nsegs = (int)segs.size();
+    if (nsegs > 1 && (r = retape(1))) return r;
+    for (int k = 0; k < nsegs; k++) {
+        if (k >= 1) BGK_HIP(hipStreamWaitEvent(c->s0, ev_tape.e[k % 2], 0));
+        if ((r = adjoint_sweep<C>(c, lin, lout, tape[k % 2], segs[k].s1 - segs[k].s0)))
+            return r;
+        BGK_HIP(hipEventRecord(ev_sweep.e[k % 2], c->s0));
+        if (k + 2 < nsegs && (r = retape(k + 2))) return r;
     }
     hipLaunchKernelGGL(k_grad_u0<C>, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, c->s0, lin,
                        u0, grad, L.nx, L.ny, L.nz, L.ystride, L.zstride);
     BGK_HIP(hipGetLastError());
+    BGK_HIP(hipStreamSynchronize(s1));
     BGK_HIP(hipStreamSynchronize(c->s0));
     const double bwd_ms = ms_since(t0);
     if (stats) {

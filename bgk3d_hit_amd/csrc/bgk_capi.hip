@@ -185,7 +185,8 @@ static int record_faces(bgk_ctx *c, hipStream_t s) {
 // intermediate S_{T+1} into slot[a] from inside the pass; a three-step pass
 // to S_{T+3} writes the faces of S_{T+1} into slot[1-a] and those of S_{T+2}
 // into slot[a] from inside the pass.
-int step_periodic(bgk_ctx *c, int nsteps, bool count) {
+int step_periodic(bgk_ctx *c, int nsteps, bool count, hipStream_t stream) {
+    hipStream_t s = stream ? stream : c->s0;
     std::vector<Box> all{full_box(c->lat)};
     const bool images = (c->flags & BGK_FLAG_REFCOMPAT_HALOS) != 0;
     const int n0 = nsteps;
@@ -197,10 +198,10 @@ int step_periodic(bgk_ctx *c, int nsteps, bool count) {
         for (; n3 > 0; n3--, nsteps -= 3) {
             const int a = c->cur;
             hipEvent_t ev;
-            BGK_HIP(timing_begin(c, TCLS_PASS3, c->s0, &ev));
+            BGK_HIP(timing_begin(c, TCLS_PASS3, s, &ev));
             BGK_HIP(launch_collide3(c->lat, a, c->omega, images ? c->gcache[1 - a] : nullptr,
-                                    images ? c->gcache[a] : nullptr, c->s0));
-            BGK_HIP(timing_end(c, TCLS_PASS3, c->s0, ev));
+                                    images ? c->gcache[a] : nullptr, s));
+            BGK_HIP(timing_end(c, TCLS_PASS3, s, ev));
             c->cur ^= 1;
         }
     }
@@ -209,23 +210,23 @@ int step_periodic(bgk_ctx *c, int nsteps, bool count) {
             const int a = c->cur;
             hipEvent_t ev;
             if (images) {
-                BGK_HIP(timing_begin(c, TCLS_HALO, c->s0, &ev));
-                BGK_HIP(launch_pack_faces(c->lat, a, c->gcache[1 - a], c->s0));
-                BGK_HIP(timing_end(c, TCLS_HALO, c->s0, ev));
+                BGK_HIP(timing_begin(c, TCLS_HALO, s, &ev));
+                BGK_HIP(launch_pack_faces(c->lat, a, c->gcache[1 - a], s));
+                BGK_HIP(timing_end(c, TCLS_HALO, s, ev));
             }
-            BGK_HIP(timing_begin(c, TCLS_PASS2, c->s0, &ev));
+            BGK_HIP(timing_begin(c, TCLS_PASS2, s, &ev));
             BGK_HIP(launch_collide2(c->lat, a, c->omega, images ? c->gcache[a] : nullptr,
-                                    c->s0));
-            BGK_HIP(timing_end(c, TCLS_PASS2, c->s0, ev));
+                                    s));
+            BGK_HIP(timing_end(c, TCLS_PASS2, s, ev));
             c->cur ^= 1;
         }
     }
     for (int t = 0; t < nsteps; t++) {
         if (images) {
-            int r = record_faces(c, c->s0);
+            int r = record_faces(c, s);
             if (r) return r;
         }
-        int r = collide_boxes(c, all, c->s0);
+        int r = collide_boxes(c, all, s);
         if (r) return r;
         c->cur ^= 1;
     }
@@ -963,6 +964,20 @@ int bgk_timing_passes(bgk_ctx *c, double t[6], int reset) {
         t[2 * k] = T.ms[cls[k]];
         t[2 * k + 1] = (double)T.launches[cls[k]];
     }
+    if (reset) timing_clear(T);
+    return BGK_OK;
+}
+
+int bgk_timing_adjoint(bgk_ctx *c, double t[4], int reset) {
+    int r = check_ctx(c);
+    if (r) return r;
+    if (!t) return fail(BGK_EINVAL, "null output");
+    BGK_HIP(harvest(c));
+    Timing &T = c->timing;
+    t[0] = T.ms[TCLS_TAPE];
+    t[1] = (double)T.launches[TCLS_TAPE];
+    t[2] = T.ms[TCLS_ADJOINT];
+    t[3] = (double)T.launches[TCLS_ADJOINT];
     if (reset) timing_clear(T);
     return BGK_OK;
 }

@@ -77,8 +77,8 @@ struct Timing {
         hipEvent_t a, b;
     };
     std::vector<Rec> recs;
-    double ms[6] = {0, 0, 0, 0, 0, 0};
-    long long launches[6] = {0, 0, 0, 0, 0, 0};
+    double ms[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    long long launches[8] = {0, 0, 0, 0, 0, 0, 0, 0};
 };
 
 }  // namespace bgk
@@ -150,9 +150,10 @@ void lattice_geometry(Lattice &L, int lx, int ly, int lz, int storage, int layou
 
 // timing helpers (no-ops unless BGK_FLAG_TIMING)
 // TCLS_PASS2 / TCLS_PASS3: collide launches of two / three time steps
-// (counted with the collide launches in bgk_timing)
+// (counted with the collide launches in bgk_timing); TCLS_TAPE / TCLS_ADJOINT:
+// the adjoint's taped forward launches and its adjoint-step launches
 enum { TCLS_COLLIDE = 0, TCLS_REDUCE = 1, TCLS_HALO = 2, TCLS_STEP = 3, TCLS_PASS2 = 4,
-       TCLS_PASS3 = 5, TCLS_N = 6 };
+       TCLS_PASS3 = 5, TCLS_TAPE = 6, TCLS_ADJOINT = 7, TCLS_N = 8 };
 hipError_t timing_begin(bgk_ctx *c, int cls, hipStream_t s, hipEvent_t *out);
 hipError_t timing_end(bgk_ctx *c, int cls, hipStream_t s, hipEvent_t begin);
 
@@ -241,6 +242,8 @@ int forward_ke(bgk_ctx *c, const double *u0_dev, int nsteps, double *J);
 
 // shared helpers used by the adjoint
 int reduce_ke(bgk_ctx *c, int copy, double *J);
-int step_periodic(bgk_ctx *c, int nsteps, bool count = true);
+// nsteps of bgk_step on the periodic layout, queued on `stream` (null: the
+// context's main stream s0)
+int step_periodic(bgk_ctx *c, int nsteps, bool count = true, hipStream_t stream = nullptr);
 
 }  // namespace bgk

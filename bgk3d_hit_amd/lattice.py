@@ -204,6 +204,14 @@ class Lattice:
         L.check(self._lib.bgk_timing_passes(self._h, _dptr(t), 1 if reset else 0))
         return {k + 1: (t[2 * k], int(t[2 * k + 1])) for k in range(3)}
 
+    def timing_adjoint(self, reset=True):
+        """Kernel time of the adjoint's taped forward passes and adjoint-step
+        passes (BGK_FLAG_TIMING): {"tape": (ms, launches), "adjoint": (ms,
+        launches)}."""
+        t = np.zeros(4)
+        L.check(self._lib.bgk_timing_adjoint(self._h, _dptr(t), 1 if reset else 0))
+        return {"tape": (t[0], int(t[1])), "adjoint": (t[2], int(t[3]))}
+
     # -- adjoint
     def adjoint_ke(self, u0, nsteps, ckpt_every=25, levels=1):
         """(J, dJ/du0, stats) for J = sum |u|^2/2 after nsteps from equilibrium(u0)."""

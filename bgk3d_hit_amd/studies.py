@@ -118,6 +118,19 @@ def adjoint_study(n=128, steps=500, ckpt=25, levels=1, k0=4.0, urms=0.02, omega=
            "adjoint_peak_bytes": st["peak_bytes"],
            "lattice_bytes": lat_bytes,
            "peak_hbm_bytes": st["peak_bytes"] + lat_bytes}
+    # kernel time of the taped forward passes (re-taping overlaps the
+    # adjoint steps on a second stream) and of the adjoint-step passes, from
+    # a second run with HIP events around every launch
+    lat = Lattice(n, n, n, omega=omega, storage=kind, timing=True)
+    try:
+        lat.adjoint_ke(u0, steps, ckpt_every=ckpt, levels=levels)
+        tk = lat.timing_adjoint()
+    finally:
+        lat.close()
+    out["taped_forward_kernel_ms"] = tk["tape"][0]
+    out["taped_forward_launches"] = tk["tape"][1]
+    out["adjoint_step_kernel_ms"] = tk["adjoint"][0]
+    out["adjoint_step_launches"] = tk["adjoint"][1]
     if log:
         log(f"[adjoint] {n}^3 x {steps} ckpt {ckpt} levels {levels}: ratio "
             f"{out['ratio_adjoint_to_forward']:.2f}, peak {out['peak_hbm_bytes'] / 2**30:.1f} GiB")

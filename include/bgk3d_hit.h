@@ -231,13 +231,25 @@ int bgk_timing(bgk_ctx *ctx, double t[8], int reset);
  * No counterpart in the reference (its timers are per section).             */
 int bgk_timing_passes(bgk_ctx *ctx, double t[6], int reset);
 
+/* With BGK_FLAG_TIMING: the kernel time of the discrete adjoint's launches
+ * (bgk_adjoint_ke): t[0], t[1] = ms and launches of its taped forward passes
+ * (the last segment in the forward run and every re-taped segment), t[2],
+ * t[3] of its adjoint-step passes.  The re-taping runs on a second stream
+ * beside the adjoint steps, so the two add up to more than the backward's
+ * wall time.  reset != 0 clears every counter.  No counterpart in the
+ * reference.                                                                */
+int bgk_timing_adjoint(bgk_ctx *ctx, double t[4], int reset);
+
 /* Discrete adjoint (not in the reference).  J = sum_x |u(x)|^2 / 2 of the
  * state reached after nsteps steps from the equilibrium of u0 (rho = 1);
  * grad = dJ/du0 (3 planes).  Checkpoints every ckpt_every steps with
- * recompute; levels = 1 or 2 (two-level checkpointing).  stats[0] = peak
- * device bytes of the adjoint's buffers, stats[1] = forward ms,
- * stats[2] = backward ms, stats[3] = number of recomputed steps.
- * u0 and grad are host arrays.                                              */
+ * recompute; levels = 1 or 2 (two-level checkpointing), 0 = tape every step.
+ * The backward sweeps the segments on the context's main stream while the
+ * next segment is re-taped from its checkpoint on the second stream.
+ * stats[0] = peak device bytes of the adjoint's buffers, stats[1] = forward
+ * ms, stats[2] = backward ms (wall), stats[3] = number of recomputed steps.
+ * The context's populations are used as work space (their contents after
+ * the call are unspecified).  u0 and grad are host arrays.                  */
 int bgk_adjoint_ke(bgk_ctx *ctx, const double *u0, int nsteps, int ckpt_every, int levels,
                    double *J, double *grad, double stats[4]);
 /* Same with u0 and grad in device memory (for framework bindings). */
