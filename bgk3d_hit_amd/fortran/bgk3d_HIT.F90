@@ -45,9 +45,45 @@ module bgk_host
   integer :: l, m, n, nprocs = 1
   integer(c_int) :: kind_c, diag_c
   type(c_ptr) :: ctx = c_null_ptr
-  ! section timers (SRC/mod_tools.F90:11-54)
+  ! section timers (SRC/mod_tools.F90:11-54): time_* from system_clock,
+  ! time_*1 from the clock of the day (the reference's two columns)
   real(dp) :: time_init = 0, time_loop = 0, time_coll = 0, time_dg = 0, time_mp = 0
+  real(dp) :: time_init1 = 0, time_loop1 = 0, time_coll1 = 0, time_dg1 = 0, time_mp1 = 0
   integer(8) :: count_rate
+contains
+  ! seconds of the day from date_and_time: the reference's second clock
+  ! (subroutine time, SRC/mod_tools.F90:46-54)
+  function tod() result(t)
+    real(dp) :: t
+    integer :: v(8)
+    call date_and_time(values=v)
+    t = 3600.0_dp * v(5) + 60.0_dp * v(6) + v(7) + v(8) / 1000.0_dp
+  end function tod
+
+  ! one line in the form of the date command to time.log (the reference
+  ! runs system("date > time.log") at start, SRC/input.F90:39, and
+  ! system("date >> time.log") at the end, SRC/finalize.F90:41-43)
+  subroutine date_line(append)
+    logical, intent(in) :: append
+    character(len=3), parameter :: wd(0:6) = ['Sun', 'Mon', 'Tue', 'Wed', 'Thu', 'Fri', 'Sat']
+    character(len=3), parameter :: mo(12) = ['Jan', 'Feb', 'Mar', 'Apr', 'May', 'Jun', &
+         'Jul', 'Aug', 'Sep', 'Oct', 'Nov', 'Dec']
+    integer, parameter :: off(12) = [0, 3, 2, 5, 0, 3, 5, 1, 4, 6, 2, 4]
+    integer :: v(8), y, dow
+    character(len=5) :: zone
+    call date_and_time(zone=zone, values=v)
+    y = v(1)
+    if (v(2) < 3) y = y - 1
+    dow = mod(y + y / 4 - y / 100 + y / 400 + off(v(2)) + v(3), 7)  ! Sakamoto
+    if (append) then
+       open(97, file='time.log', status='unknown', position='append')
+    else
+       open(97, file='time.log', status='replace')
+    end if
+    write(97, '(a3,1x,a3,1x,i2,1x,i2.2,":",i2.2,":",i2.2,1x,a5,1x,i4)') wd(dow), mo(v(2)), &
+         v(3), v(5), v(6), v(7), zone, v(1)
+    close(97)
+  end subroutine date_line
 end module bgk_host
 
 program bgk_HIT
@@ -55,6 +91,7 @@ program bgk_HIT
   implicit none
   integer :: itime, itstart, nrun
   integer(8) :: c0, c1, e0, e1
+  real(dp) :: t0, t1
   character(len=256) :: deck
   logical, external :: step_event
 
@@ -62,6 +99,7 @@ program bgk_HIT
   deck = 'bgk.input'
   if (command_argument_count() >= 1) call get_command_argument(1, deck)
   call system_clock(c0, count_rate)
+  t0 = tod()
   call input(trim(deck))
   call hencol
   call outdat
@@ -70,9 +108,11 @@ program bgk_HIT
   if (irestart == 1) call read_restart(itstart)
   call system_clock(c1)
   time_init = real(c1 - c0, dp) / real(count_rate, dp)
+  time_init1 = tod() - t0
 
   call vtk_slices(itstart - 1)                    ! SRC/bgk3d_HIT.F90:118-120
   call system_clock(e0)
+  t0 = tod()
   itime = itstart
   do while (itime <= itfin)                      ! SRC/bgk3d_HIT.F90:127-146
      call boundaries(itime)
@@ -97,6 +137,7 @@ program bgk_HIT
   call bgk_check(bgk_synchronize(ctx), 'bgk_synchronize')
   call system_clock(e1)
   time_loop = real(e1 - e0, dp) / real(count_rate, dp)
+  time_loop1 = tod() - t0
   call finalize(itstart)
 end program bgk_HIT
 
@@ -109,6 +150,7 @@ subroutine input(deck)
   namelist /parameters/ svisc, u0, itfin, ivtim, isignal, itsave, icheck, irestart, &
        lx, ly, lz, border, ipad, jpad, proc_x, proc_y, proc_z, cteS, &
        init, storage, diag, k0, urms, seed, omega
+  call date_line(.false.)  ! SRC/input.F90:39
   open(15, file=deck, status='old', iostat=ios)
   if (ios /= 0) then
      write(6, *) 'ERROR: cannot open ', deck
@@ -178,6 +220,29 @@ subroutine outdat
   write(16, *) 'INFO: svisc, omega, u0 ', svisc, omega, u0
   write(16, *) 'INFO: itfin, icheck, isignal ', itfin, icheck, isignal
   write(6, *) 'INFO: lattice ', lx, ly, lz, ' storage ', trim(storage), ' omega ', omega
+  ! the rank's task log (SRC/setup_MPI.F90:260-323): the corners of its
+  ! block and its neighbours, all rank 0 here (one rank, periodic)
+  open(38, file='task.000000.log', status='unknown')
+  write(38, *) 0, 0, 0
+  write(38, *) 0, 0, n
+  write(38, *) " "
+  write(38, *) l, 0, 0
+  write(38, *) l, 0, n
+  write(38, *) " "
+  write(38, *) l, m, 0
+  write(38, *) l, m, n
+  write(38, *) " "
+  write(38, *) 0, m, 0
+  write(38, *) 0, m, n
+  write(38, *) " "
+  write(38, *) 0, 0, 0
+  write(38, *) 0, 0, n
+  write(38, *) "# my front task is ", 0
+  write(38, *) "# my rear task is  ", 0
+  write(38, *) "# my left task is  ", 0
+  write(38, *) "# my right task is ", 0
+  write(38, *) "# my up task is    ", 0
+  write(38, *) "# my down task is  ", 0
 end subroutine outdat
 
 ! SRC/alloca.F90:178-216: the device lattice
@@ -250,10 +315,13 @@ subroutine boundaries(itime)
   implicit none
   integer, intent(in) :: itime
   integer(8) :: c0, c1
+  real(dp) :: t0
   call system_clock(c0)
+  t0 = tod()
   call bgk_check(bgk_boundaries(ctx), 'bgk_boundaries')
   call system_clock(c1)
   time_mp = time_mp + real(c1 - c0, dp) / real(count_rate, dp)
+  time_mp1 = time_mp1 + (tod() - t0)
   if (itime < 0) write(6, *) itime
 end subroutine boundaries
 
@@ -263,11 +331,14 @@ subroutine col_MC(itime)
   implicit none
   integer, intent(in) :: itime
   integer(8) :: c0, c1
+  real(dp) :: t0
   call system_clock(c0)
+  t0 = tod()
   call bgk_check(bgk_col_MC(ctx), 'bgk_col_MC')
   call bgk_check(bgk_synchronize(ctx), 'bgk_synchronize')
   call system_clock(c1)
   time_coll = time_coll + real(c1 - c0, dp) / real(count_rate, dp)
+  time_coll1 = time_coll1 + (tod() - t0)
   if (itime < 0) write(6, *) itime
 end subroutine col_MC
 
@@ -279,11 +350,14 @@ subroutine col_MCn(itime, nrun)
   implicit none
   integer, intent(in) :: itime, nrun
   integer(8) :: c0, c1
+  real(dp) :: t0
   call system_clock(c0)
+  t0 = tod()
   call bgk_check(bgk_step(ctx, int(nrun, c_int)), 'bgk_step')
   call bgk_check(bgk_synchronize(ctx), 'bgk_synchronize')
   call system_clock(c1)
   time_coll = time_coll + real(c1 - c0, dp) / real(count_rate, dp)
+  time_coll1 = time_coll1 + (tod() - t0)
   if (itime < 0) write(6, *) itime
 end subroutine col_MCn
 
@@ -302,9 +376,11 @@ subroutine diagnostic(itime)
   implicit none
   integer, intent(in) :: itime
   integer(8) :: c0, c1
+  real(dp) :: t0
   real(c_double) :: ds(7)
   if (mod(itime, icheck) == 0) then
      call system_clock(c0)
+     t0 = tod()
      ! diagno and dissipation (SRC/diagnostic.F90:47-49) from one pass over
      ! the populations, bit-identical to the two calls
      call bgk_check(bgk_diagnostics(ctx, real(svisc, c_double), diag_c, ds), &
@@ -315,6 +391,7 @@ subroutine diagnostic(itime)
      if (mod(itime, ivtim) == 0) call vtk_slices(itime)
      call system_clock(c1)
      time_dg = time_dg + real(c1 - c0, dp) / real(count_rate, dp)
+     time_dg1 = time_dg1 + (tod() - t0)
   end if
   if (mod(itime, isignal) == 0) then
      write(6, '(a,i8,a,es12.4,a)') ' Iteration ', itime, '  mean step time ', &
@@ -542,40 +619,91 @@ subroutine read_restart(itstart)
   write(6, *) 'INFO: restarted from step ', rit
 end subroutine read_restart
 
-! SRC/finalize.F90:44-103
+! SRC/finalize.F90:23-118
 subroutine finalize(itstart)
   use bgk_host
   implicit none
   integer, intent(in) :: itstart
-  real(dp) :: glups
+  real(dp) :: glups, glups1, knorm, r(2)
   real(c_double) :: t(8)
-  ! SRC/finalize.F90:83-85 (itfin - itstart steps in the numerator)
+  call date_line(.true.)
+  ! per-rank section timers and ratios (SRC/finalize.F90:45-63); one rank
+  ! here, so task.000000.log
+  write(38, *) "# Time for section "
+  write(38, 1100) time_init, time_init1
+  write(38, 1101) time_loop, time_loop1
+  write(38, 1102) time_coll, time_coll1
+  write(38, 1103) time_dg, time_dg1
+  write(38, 1115) time_mp, time_mp1
+  write(38, 9999)
+  write(38, *) "# Ratio "
+  write(38, 1202) ratio(time_coll, time_loop), ratio(time_coll1, time_loop1)
+  write(38, 1203) ratio(time_dg, time_loop), ratio(time_dg1, time_loop1)
+  write(38, 1215) ratio(time_mp, time_loop), ratio(time_mp1, time_loop1)
+  r(1) = ratio(time_loop - time_coll - time_dg - time_mp, time_loop)
+  r(2) = ratio(time_loop1 - time_coll1 - time_dg1 - time_mp1, time_loop1)
+  write(38, 1216) r
+  write(38, 9999)
+  ! exchange time and bandwidth per axis (SRC/finalize.F90:58-63): the one
+  ! GPU exchanges nothing across x and y (periodic wrap in the index math),
+  ! and its z exchange (slab ghosts, reference halo images) is the
+  ! boundaries time
+  knorm = real(storage_bytes(), dp) / (1024.0_dp * 1024.0_dp)
+  write(38, *) "# Z-MPI time, BW (MB/s) -->", time_mp, &
+       ratio(real(m + 2, dp) * real(l + 2, dp) * real(itfin, dp) * knorm, time_mp)
+  write(38, *) "# Y-MPI time, BW (MB/s) -->", 0.0_dp, 0.0_dp
+  write(38, *) "# X-MPI time, BW (MB/s) -->", 0.0_dp, 0.0_dp
+  close(38)
+  ! SRC/finalize.F90:75-90 (GLUPs of bgk.perf on the second clock, of
+  ! stdout on the first, itfin - itstart steps in the numerator)
   glups = real(lx, dp) * real(ly, dp) * real(lz, dp) * real(itfin - itstart, dp) / &
        (time_loop * 1000 * 1000 * 1000)
+  glups1 = real(lx, dp) * real(ly, dp) * real(lz, dp) * real(itfin - itstart, dp) / &
+       (max(time_loop1, 1e-9_dp) * 1000 * 1000 * 1000)
   open(69, file='bgk.perf', status='unknown')
   write(69, 9999)
-  write(69, 1100) time_init
-  write(69, 1101) time_loop
-  write(69, 1102) time_coll
-  write(69, 1103) time_dg
-  write(69, 1115) time_mp
+  write(69, 1100) time_init, time_init1
+  write(69, 1101) time_loop, time_loop1
+  write(69, 1102) time_coll, time_coll1
+  write(69, 1103) time_dg, time_dg1
+  write(69, 1115) time_mp, time_mp1
   write(69, 9999)
-  write(69, 3000) glups
+  write(69, 3000) glups1
+  write(69, 1216) r
   write(69, 9999)
   close(69)
   write(6, 9999)
-  write(6, 2000) time_loop
+  write(6, *) "INFO: That's all Folks!!!!!! "
+  write(6, 2000) time_loop, time_loop1
   write(6, 3000) glups
+  write(6, *) "INFO: That's all Folks!!!!!! "
   write(6, 9999)
   call bgk_check(bgk_timing(ctx, t, 1_c_int), 'bgk_timing')
   call bgk_check(bgk_destroy(ctx), 'bgk_destroy')
   close(16)
 9999 format(" #--------------------------------")
-1100 format(" # init   time", 1(e14.6, 1x))
-1101 format(" # loop   time", 1(e14.6, 1x))
-1102 format(" # comp.  time", 1(e14.6, 1x))
-1103 format(" # diag.  time", 1(e14.6, 1x))
-1115 format(" # MPI    time", 1(e14.6, 1x))
-2000 format(" # loop time (s) ", e14.6)
-3000 format(" # GLUPs ", e14.6)
+1100 format(" # init   time", 2(e14.6, 1x))
+1101 format(" # loop   time", 2(e14.6, 1x))
+1102 format(" # comp.  time", 2(e14.6, 1x))
+1103 format(" # diag.  time", 2(e14.6, 1x))
+1115 format(" # MPI    time", 2(e14.6, 1x))
+1202 format(" # Ratio Coll ", 2(f7.3, 1x))
+1203 format(" # Ratio Dg.  ", 2(f7.3, 1x))
+1215 format(" # Ratio MPI  ", 2(f7.3, 1x))
+1216 format(" # Check      ", 2(f7.3, 1x))
+2000 format(" # loop   time", 2(e14.6, 1x))
+3000 format(" # GLUPs      ", 1(e14.6))
+contains
+  real(dp) function ratio(a, b)
+    real(dp), intent(in) :: a, b
+    ratio = 0.0_dp
+    if (b > 0.0_dp) ratio = a / b
+  end function ratio
+  integer function storage_bytes()
+    select case (trim(storage))
+    case ('f64'); storage_bytes = 8
+    case ('f16', 'bf16'); storage_bytes = 2
+    case default; storage_bytes = 4
+    end select
+  end function storage_bytes
 end subroutine finalize

@@ -187,3 +187,93 @@ def test_driver_refcompat_fused_passes(gpu, tmp_path):
             assert ds[row, 2] == s[1] and ds[row, 3] == s[0], (it, ds[row], s)
             row += 1
     assert row == itfin // icheck
+
+
+def fortran_e(x, d=6, w=14):
+    """Fortran Ew.d output of a double: sign, 0.<d digits>E<sign><2 digits>,
+    right-justified in w characters."""
+    if x == 0.0:
+        s = "0." + "0" * d + "E+00"
+    else:
+        m = f"{abs(x):.{d - 1}E}"            # d.ddddd E+xx, correctly rounded
+        digits = m[0] + m[2:d + 1]
+        s = ("-" if x < 0 else "") + "0." + digits + f"E{int(m.split('E')[1]) + 1:+03d}"
+    return s.rjust(w)
+
+
+def fixed_fields(line, widths):
+    out, i = [], 0
+    for wd in widths:
+        out.append(line[i:i + wd])
+        i += wd
+    return out
+
+
+def test_fortran_e_format_examples():
+    assert fortran_e(0.0012345678) == "  0.123457E-02"
+    assert fortran_e(-98765.4321) == " -0.987654E+05"
+    assert fortran_e(1.0) == "  0.100000E+01"
+    assert fortran_e(0.0) == "  0.000000E+00"
+    assert fortran_e(0.9999999) == "  0.100000E+01"
+
+
+@pytest.mark.gpu
+def test_driver_reference_format_outputs(gpu, tmp_path):
+    """The files post-processing reads, in the reference's formats:
+    diagno.dat (i8, 5(e14.6,1x); SRC/diagno.F90:137,152), dissipation.dat
+    (i8, 4(e14.6,1x); SRC/dissipation.F90:265-278) with the values of the
+    full-precision side files rounded as Fortran's e14.6 rounds them;
+    bgk.perf with two timer columns, GLUPs and the check line
+    (SRC/finalize.F90:75-90); task.000000.log section timers, ratios and
+    per-axis exchange lines (SRC/finalize.F90:44-63); time.log with the
+    start and end dates (SRC/input.F90:39, SRC/finalize.F90:41-43)."""
+    n, svisc, itfin, icheck, u0 = 24, 0.01, 60, 20, 0.02
+    deck = (f"&parameters\nlx={n}\nly={n}\nlz={n}\nsvisc={svisc}\nu0={u0}\nitfin={itfin}\n"
+            f"icheck={icheck}\ninit='hit'\nk0=3.0\nurms=0.02\nseed=77\n"
+            f"storage='f64'\ndiag='refcompat'\n/\n")
+    print("[gpu-case] fortran driver reference formats", flush=True)
+    p = run_driver(tmp_path, deck)
+    assert "That's all Folks" in p.stdout
+    full_d = np.loadtxt(tmp_path / "diagno.full.dat", ndmin=2)
+    full_s = np.loadtxt(tmp_path / "dissipation.full.dat", ndmin=2)
+    dlines = (tmp_path / "diagno.dat").read_text().splitlines()
+    slines = (tmp_path / "dissipation.dat").read_text().splitlines()
+    assert len(dlines) == len(slines) == itfin // icheck == full_d.shape[0]
+    for row, line in enumerate(dlines):
+        f = fixed_fields(line, [8] + [14, 1] * 5)
+        assert int(f[0]) == full_d[row, 0]
+        for k in range(5):
+            assert f[1 + 2 * k] == fortran_e(full_d[row, 1 + k]), (row, k, line)
+            # 1x between fields (a trailing 1x ends the record unwritten)
+            assert f[2 + 2 * k] == (" " if k < 4 else f[2 + 2 * k]) and f[2 + 2 * k] in (" ", "")
+    for row, line in enumerate(slines):
+        f = fixed_fields(line, [8] + [14, 1] * 4)
+        it, t, tke, diss = full_s[row]
+        assert int(f[0]) == it
+        # tke / nprocs / (l m n) / u0^2 in the driver's operation order
+        norm = (tke / 1.0) / float(n) / float(n) / float(n) / (u0 * u0)
+        assert [f[1], f[3], f[5]] == [fortran_e(t), fortran_e(norm), fortran_e(diss)], line
+    perf = (tmp_path / "bgk.perf").read_text().splitlines()
+    names = ["init", "loop", "comp.", "diag.", "MPI"]
+    timers = {}
+    for line in perf:
+        lab = line[:14].split()
+        if len(lab) == 3 and lab[0] == "#" and lab[1] in names and lab[2] == "time":
+            v = fixed_fields(line, [14, 14, 1, 14, 1])
+            timers[lab[1]] = (float(v[1]), float(v[3]))
+    assert set(timers) == set(names), perf
+    glups = [line for line in perf if line.startswith(" # GLUPs")]
+    check = [line for line in perf if line.startswith(" # Check")]
+    assert len(glups) == 1 and len(check) == 1, perf
+    loop1 = timers["loop"][1]
+    if loop1 > 0:  # the GLUPs of bgk.perf use the second (time-of-day) clock
+        g = float(glups[0][14:])
+        assert abs(g - n ** 3 * (itfin - 1) / (loop1 * 1e9)) <= 1e-5 * g + 1e-12
+    c = fixed_fields(check[0], [14, 7, 1, 7])
+    assert -1.0 <= float(c[1]) <= 1.0
+    task = (tmp_path / "task.000000.log").read_text()
+    for key in ("# Time for section", "# Ratio", "# Ratio Coll", "# Check",
+                "# Z-MPI time, BW (MB/s)", "# my up task is"):
+        assert key in task, key
+    dates = (tmp_path / "time.log").read_text().splitlines()
+    assert len(dates) == 2 and all(len(x.split()) == 6 for x in dates), dates
