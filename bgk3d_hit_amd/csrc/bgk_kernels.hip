@@ -18,6 +18,7 @@
 // All reductions accumulate in FP64 with a fixed grid and a fixed
 // combination order, so they are deterministic run to run.
 #include <algorithm>
+#include <type_traits>
 
 #include "bgk_collide.h"
 
@@ -292,13 +293,24 @@ __device__ __forceinline__ void reduce_body(const Pops<S> &P, const S *gcache, c
     if (WHAT & 2) block_sum_store<RED_NV>(vs, ps + (size_t)t * RED_NV);
 }
 
+// waves per SIMD the reduction kernels are compiled for (the second
+// argument of __launch_bounds__): four for FP32 storage and compute reading
+// the state itself (at most 128 VGPRs, two 8-wave workgroups per CU: 512^3
+// diagno+dissipation 2.41 against 2.67 ms), one (the compiler's choice)
+// otherwise (FP16/BF16 gained nothing, FP64 and the face-image variants
+// would spill)
+template <typename S, typename C, int GC>
+constexpr int red_waves() {
+    return (std::is_same<S, float>::value && std::is_same<C, float>::value && GC == 0) ? 4 : 1;
+}
+
 template <typename S, typename C>
-__global__ __launch_bounds__(Tile<S>::X * Tile<S>::Y) void k_diagno(Pops<S> P, RedGeo g, C cte1,
+__global__ __launch_bounds__(Tile<S>::X * Tile<S>::Y, (red_waves<S, C, 0>())) void k_diagno(Pops<S> P, RedGeo g, C cte1,
                                                                     double *partials) {
     reduce_body<S, C, 1, 0>(P, nullptr, g, cte1, (C)0, partials, nullptr);
 }
 template <typename S, typename C, int GC>
-__global__ __launch_bounds__(Tile<S>::X * Tile<S>::Y) void k_dissipation(Pops<S> P,
+__global__ __launch_bounds__(Tile<S>::X * Tile<S>::Y, (red_waves<S, C, GC>())) void k_dissipation(Pops<S> P,
                                                                          const S *gcache,
                                                                          RedGeo g, C cte1,
                                                                          C svisc,
@@ -307,7 +319,7 @@ __global__ __launch_bounds__(Tile<S>::X * Tile<S>::Y) void k_dissipation(Pops<S>
 }
 // both in one pass over the populations (bgk_diagnostics)
 template <typename S, typename C, int GC>
-__global__ __launch_bounds__(Tile<S>::X * Tile<S>::Y) void k_diag_diss(Pops<S> P,
+__global__ __launch_bounds__(Tile<S>::X * Tile<S>::Y, (red_waves<S, C, GC>())) void k_diag_diss(Pops<S> P,
                                                                        const S *gcache,
                                                                        RedGeo g, C cte1,
                                                                        C svisc, double *pd,
