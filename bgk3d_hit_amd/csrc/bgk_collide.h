@@ -883,8 +883,7 @@ __global__ __launch_bounds__(collide3_threads(TX, TY), 1) void k_collide3(Pops<S
         if (active) {  // S1 at plane z
             C b[19], ux, uy, uz;
             collide_site<C, SH>(f, b, omega, cte1, ux, uy, uz);
-#pragma unroll
-            for (int q = 0; q < 19; q++) s1[q] = Storage<S>::template store<C>(b[q]);
+            store_all<S, C>(b, s1);
 #pragma unroll
             for (int q = 0; q < 19; q++)
                 if (pgroup<1>(q) == 0) l1[pbase(0) + pslot<1>(q)][ey][ex] = s1[q];
@@ -897,8 +896,7 @@ __global__ __launch_bounds__(collide3_threads(TX, TY), 1) void k_collide3(Pops<S
             for (int q = 0; q < 19; q++)
                 f2[q] = Storage<S>::template load<C>(pull(l1, q, ey, ex, pb, s1[QN], r1O_2, r1R_1));
             collide_site<C, SH>(f2, b, omega, cte1, ux, uy, uz);
-#pragma unroll
-            for (int q = 0; q < 19; q++) s2[q] = Storage<S>::template store<C>(b[q]);
+            store_all<S, C>(b, s2);
 #pragma unroll
             for (int q = 0; q < 19; q++)
                 if (pgroup<1>(q) == 0) l2[pbase(0) + pslot<1>(q)][ey - 1][ex - 1] = s2[q];
@@ -924,12 +922,24 @@ __global__ __launch_bounds__(collide3_threads(TX, TY), 1) void k_collide3(Pops<S
                 f3[q] = Storage<S>::template load<C>(
                     pull(l2, q, ey - 1, ex - 1, pb, s2[QN], r2O_2, r2R_1));
             collide_site<C, SH>(f3, b, omega, cte1, ux, uy, uz);
-            const __amdgpu_buffer_rsrc_t R =
-                raw_rsrc(dst.p[0] + (ptrdiff_t)(wrapz(z - 2) + zoff) * plane);
+            S s3[19];
+            store_all<S, C>(b, s3);
+            const ptrdiff_t zo = (ptrdiff_t)(wrapz(z - 2) + zoff) * plane;
+            if constexpr (sizeof(S) == 2) {
+                // global stores from per-population scalar bases: a value in
+                // the high half of a packed pair stores with the d16_hi form
+                // (the offset opaque per plane, so that it stays the 32-bit
+                // offset of the scalar-base store form, see two_step_pass)
+                uint32_t c = X[1] + Y[1];
+                asm volatile("" : "+v"(c));
 #pragma unroll
-            for (int q = 0; q < 19; q++)
-                buf_st<S>(Storage<S>::template store<C>(b[q]), R, X[1] + Y[1],
-                          (uint32_t)q * g.pstride);
+                for (int q = 0; q < 19; q++) st_at(plane_ptr(dst.p[q], zo), c, s3[q]);
+            } else {
+                const __amdgpu_buffer_rsrc_t R = raw_rsrc(dst.p[0] + zo);
+#pragma unroll
+                for (int q = 0; q < 19; q++)
+                    buf_st<S>(s3[q], R, X[1] + Y[1], (uint32_t)q * g.pstride);
+            }
         }
         lds_barrier();
         if (do2) {

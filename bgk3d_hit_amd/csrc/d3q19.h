@@ -12,6 +12,8 @@
 // on any IEEE machine that evaluates the same expressions.
 #pragma once
 
+#include <type_traits>
+
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -102,6 +104,36 @@ struct Storage<bf16_t> {
         return bf16_t{__builtin_bit_cast(uint16_t, (__bf16)f)};
     }
 };
+
+// The storage values of b[0..18] into out[]: for the 16-bit kinds two
+// values per conversion instruction (v_cvt_pk_f16_f32 / v_cvt_pk_bf16_f32:
+// each f32 value rounded to nearest even exactly as by the one-value form,
+// the pair packed into one register, whose halves the 16-bit stores and the
+// widening conversions address directly); element by element otherwise.
+// The conversions are written as asm: the compiler lowered the same pair
+// conversion written as a two-element vector conversion to one converted
+// element stored twice, and the asm also keeps the f32 inputs opaque (see
+// Storage<half_t>::store).
+template <typename S, typename C>
+__device__ __forceinline__ void store_all(const C (&b)[19], S (&out)[19]) {
+    if constexpr (std::is_same<S, half_t>::value || std::is_same<S, bf16_t>::value) {
+#pragma unroll
+        for (int q = 0; q + 1 < 19; q += 2) {
+            const float lo = (float)b[q], hi = (float)b[q + 1];
+            uint32_t p;
+            if constexpr (std::is_same<S, half_t>::value)
+                asm("v_cvt_pk_f16_f32 %0, %1, %2" : "=v"(p) : "v"(lo), "v"(hi));
+            else
+                asm("v_cvt_pk_bf16_f32 %0, %1, %2" : "=v"(p) : "v"(lo), "v"(hi));
+            out[q].bits = (uint16_t)p;
+            out[q + 1].bits = (uint16_t)(p >> 16);
+        }
+        out[18] = Storage<S>::template store<C>(b[18]);
+    } else {
+#pragma unroll
+        for (int q = 0; q < 19; q++) out[q] = Storage<S>::template store<C>(b[q]);
+    }
+}
 
 // ---------------------------------------------------------------- collide
 // Lattice constants in the compute kind: the reference casts double
