@@ -171,17 +171,26 @@ __device__ __forceinline__ S buf_ld(__amdgpu_buffer_rsrc_t r, uint32_t voff, uin
     else
         return __builtin_bit_cast(S, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
 }
+// Cache policy of the population stores: the sc1 bit (bit 4 of the
+// auxiliary operand).  A pass never reads back what it stores, and with sc1
+// the stores displace less of the L2's read working set (the halo rows and
+// planes that neighbouring workgroups read again): measured on the
+// three-step pass at 512^3 FP32, read traffic 16.2 -> 14.0 GB per launch
+// and 1-3% less time at FP32 and FP64; values are unchanged.  nt (bit 1)
+// cut reads further (13.2 GB) but cost 4% time.
+constexpr int POP_STORE_POLICY = 16;
 template <typename S>
 __device__ __forceinline__ void buf_st(S v, __amdgpu_buffer_rsrc_t r, uint32_t voff,
                                        uint32_t soff) {
+    constexpr int A = POP_STORE_POLICY;
     if constexpr (sizeof(S) == 2)
-        __builtin_amdgcn_raw_buffer_store_b16(__builtin_bit_cast(uint16_t, v), r, voff, soff, 0);
+        __builtin_amdgcn_raw_buffer_store_b16(__builtin_bit_cast(uint16_t, v), r, voff, soff, A);
     else if constexpr (sizeof(S) == 4)
-        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), r, voff, soff, 0);
+        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), r, voff, soff, A);
     else
         __builtin_amdgcn_raw_buffer_store_b64(
             __builtin_bit_cast(unsigned int __attribute__((ext_vector_type(2))), v), r, voff,
-            soff, 0);
+            soff, A);
 }
 
 // whether the in-plane byte offsets (x + y ystride, plus the population
