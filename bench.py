@@ -8,10 +8,15 @@ The headline `value` is the FP32 run (storage and compute FP32); the FP64
 and reduced-storage (FP16 / BF16 shifted f_i - w_i, FP32 collide) runs of
 the same workload are reported under `variants`.
 
-bgk_step advances lattices this size three time steps per launch
-(k_collide3: the first two steps' states kept in LDS) and takes a two-step
-pass (k_collide2) for what is left of each 50-step block; the two-step pass
-alone and the one-pass-per-step kernel (k_collide) are reported beside it.
+Each block of steps and its reductions is one bgk_step_diagnostics call:
+lattices this size advance three time steps per launch (k_collide3: the
+first two steps' states kept in LDS), and the last two steps of a block
+(50 = 16 * 3 + 2) share one pass with the diagnostics of the state they
+leave (k_collide3's RED form: two steps, then diagno and the pulled
+dissipation sums in place of a third collision) instead of a two-step pass
+followed by a reduction pass.  The two-step pass alone and the
+one-pass-per-step kernel (k_collide) are reported beside it (their blocks
+end in a separate reduction pass).
 
 `roofline` is measured live for the dominant kernel (the launch depth with
 the most collide time): HIP events around every collide launch on the stream
@@ -79,13 +84,14 @@ def run_gpu(kind, n, steps, warmup, diag_every, omega=1.95, shape=None, layout="
         # ceil(steps / diag_every) diagnostics passes in the timed region
         while done < steps:
             k = min(diag_every, steps - done)
-            lat.step(k)
+            # k steps, then diagno + dissipation of the state they leave
+            d5, s2 = lat.step_diagnostics(k, nu)
             done += k
-            d5, s2 = lat.diagnostics(nu)  # diagno + dissipation, one pass
             diag.append((done + warmup, *d5, *s2))
         lat.synchronize()
         wall = time.perf_counter() - t0
         passes = lat.timing_passes(reset=False)
+        fused = lat.timing_fused(reset=False)
         t = lat.timing(reset=True)
     finally:
         lat.close()
@@ -120,6 +126,8 @@ def run_gpu(kind, n, steps, warmup, diag_every, omega=1.95, shape=None, layout="
         "collide_launches": t["collide_launches"],
         "halo_ms_per_step": t["halo_ms"] / steps,
         "reduce_ms_total": t["reduce_ms"],
+        "fused_pass_avg_ms": fused[0] / max(fused[1], 1),
+        "fused_passes": fused[1],
         "achieved_gbs": achieved,
         "frac": achieved / HBM_PEAK_GBS,
         "algo_bytes_per_launch": algo_bytes,
@@ -448,6 +456,8 @@ def main():
                                f"and after the last step",
                    "lattice": [a.size] * 3, "omega": 1.95, "diag_every": a.diag_every,
                    "reductions_in_timed_region": main_run["reductions_timed"],
+                   "reductions_fused_with_last_two_steps": main_run["fused_passes"],
+                   "fused_pass_avg_ms": round(main_run["fused_pass_avg_ms"], 4),
                    "layout": "per copy [z][y][q][x] (19 population rows of a y line side by "
                              "side), halo-free, periodic wrap in index math"},
         "roofline": roofline(main_run),

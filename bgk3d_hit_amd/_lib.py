@@ -53,6 +53,7 @@ SIGNATURES = {
     "bgk_diagno": [_p, _c_int, _dp],
     "bgk_dissipation": [_p, _c_double, _c_int, _dp],
     "bgk_diagnostics": [_p, _c_double, _c_int, _dp],
+    "bgk_step_diagnostics": [_p, _c_int, _c_double, _dp],
     "bgk_diagnostics_noalloc": [_p, _c_int, _c_int, _c_int, _c_int, _c_int, _c_double, _p, _p],
     "bgk_get_box": [_p, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _dp],
     "bgk_checksum": [_p, ctypes.POINTER(ctypes.c_uint64)],
@@ -60,6 +61,7 @@ SIGNATURES = {
     "bgk_timing": [_p, _dp, _c_int],
     "bgk_timing_passes": [_p, _dp, _c_int],
     "bgk_timing_adjoint": [_p, _dp, _c_int],
+    "bgk_timing_fused": [_p, _dp, _c_int],
     "bgk_adjoint_ke": [_p, _dp, _c_int, _c_int, _c_int, _dp, _dp, _dp],
     "bgk_adjoint_ke_device": [_p, _p, _c_int, _c_int, _c_int, _dp, _p, _dp],
     "bgk_forward_ke": [_p, _dp, _c_int, _dp],

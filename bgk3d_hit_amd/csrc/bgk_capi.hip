@@ -881,6 +881,46 @@ int bgk_diagnostics(bgk_ctx *c, double svisc, int mode, double out[7]) {
     return dissipation_finish(c->red_host + 8, out + 5);
 }
 
+// The reference's loop body at a diagnostic step (SRC/bgk3d_HIT.F90:127-146:
+// the time steps, then diagno and dissipation of the state they leave).  On
+// the periodic lattice, where three-step passes run and no halo images are
+// kept, and for step counts of the form 3k + 2 or 3k + 1 (k >= 1), the last
+// two steps and the reductions share one pass (launch_collide3_diag); 3k
+// steps keep the separate reduction pass (a fused pass there would turn the
+// last three-step pass into a two-step and a fused pass, more work than the
+// reduction it saves).
+int bgk_step_diagnostics(bgk_ctx *c, int nsteps, double svisc, double out[7]) {
+    int r = check_ctx(c);
+    if (r) return r;
+    if (!out) return fail(BGK_EINVAL, "null output");
+    if (nsteps < 0) return fail(BGK_EINVAL, "nsteps must be >= 0");
+    const Lattice &L = c->lat;
+    const long long slots = std::max(c->red_blocks, reduction_parts(L));
+    const bool fused = L.layout == BGK_LAYOUT_PERIODIC &&
+                       !(c->flags & BGK_FLAG_REFCOMPAT_HALOS) && nsteps % 3 != 0 &&
+                       (nsteps % 3 == 2 || nsteps >= 4) && three_step_ok(L) &&
+                       collide3_diag_parts(L) > 0 && collide3_diag_parts(L) <= slots;
+    if (!fused) {
+        if ((r = bgk_step(c, nsteps))) return r;
+        return bgk_diagnostics(c, svisc, BGK_DIAG_CONSISTENT, out);
+    }
+    hipEvent_t ev, ev2;
+    BGK_HIP(timing_begin(c, TCLS_STEP, c->s0, &ev));
+    if ((r = step_periodic(c, nsteps - 2))) return r;
+    BGK_HIP(timing_begin(c, TCLS_PASS2D, c->s0, &ev2));
+    BGK_HIP(launch_collide3_diag(L, c->cur, c->omega, svisc, c->red_partials, c->red_out,
+                                 c->s0));
+    BGK_HIP(timing_end(c, TCLS_PASS2D, c->s0, ev2));
+    c->cur ^= 1;
+    c->steps += 2;
+    BGK_HIP(timing_end(c, TCLS_STEP, c->s0, ev));
+    BGK_HIP(hipMemcpyAsync(c->red_host, c->red_out, 16 * sizeof(double), hipMemcpyDeviceToHost,
+                           c->s0));
+    BGK_HIP(hipStreamSynchronize(c->s0));
+    if ((r = diagno_finish(L, BGK_DIAG_CONSISTENT, c->red_host, out))) return r;
+    return dissipation_finish(c->red_host + 8, out + 5);
+}
+
 int bgk_get_box(bgk_ctx *c, int x0, int x1, int y0, int y1, int z0, int z1, double *out) {
     int r = check_ctx(c);
     if (r) return r;
@@ -941,8 +981,9 @@ int bgk_timing(bgk_ctx *c, double t[8], int reset) {
     if (!t) return fail(BGK_EINVAL, "null output");
     BGK_HIP(harvest(c));
     Timing &T = c->timing;
-    t[0] = T.ms[TCLS_COLLIDE] + T.ms[TCLS_PASS2] + T.ms[TCLS_PASS3];
-    t[1] = (double)(T.launches[TCLS_COLLIDE] + T.launches[TCLS_PASS2] + T.launches[TCLS_PASS3]);
+    t[0] = T.ms[TCLS_COLLIDE] + T.ms[TCLS_PASS2] + T.ms[TCLS_PASS3] + T.ms[TCLS_PASS2D];
+    t[1] = (double)(T.launches[TCLS_COLLIDE] + T.launches[TCLS_PASS2] +
+                    T.launches[TCLS_PASS3] + T.launches[TCLS_PASS2D]);
     t[2] = T.ms[TCLS_REDUCE];
     t[3] = (double)T.launches[TCLS_REDUCE];
     t[4] = T.ms[TCLS_HALO];
@@ -964,6 +1005,18 @@ int bgk_timing_passes(bgk_ctx *c, double t[6], int reset) {
         t[2 * k] = T.ms[cls[k]];
         t[2 * k + 1] = (double)T.launches[cls[k]];
     }
+    if (reset) timing_clear(T);
+    return BGK_OK;
+}
+
+int bgk_timing_fused(bgk_ctx *c, double t[2], int reset) {
+    int r = check_ctx(c);
+    if (r) return r;
+    if (!t) return fail(BGK_EINVAL, "null output");
+    BGK_HIP(harvest(c));
+    Timing &T = c->timing;
+    t[0] = T.ms[TCLS_PASS2D];
+    t[1] = (double)T.launches[TCLS_PASS2D];
     if (reset) timing_clear(T);
     return BGK_OK;
 }

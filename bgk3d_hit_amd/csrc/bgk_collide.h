@@ -18,6 +18,34 @@ struct Pops {
     S *p[19];
 };
 
+// Reduction partials: 8 doubles per workgroup slot.
+constexpr int RED_NV = 8;
+
+// Reduce NV doubles over the block (any size up to 1024 threads, all threads
+// taking part) and write them to out[0..NV).
+template <int NV>
+__device__ __forceinline__ void block_sum_store(double (&v)[NV], double *out) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+#pragma unroll
+        for (int k = 0; k < NV; k++) v[k] += __shfl_xor(v[k], off);
+    }
+    __shared__ double sm[16][NV];
+    const int tid = (int)(threadIdx.x + blockDim.x * threadIdx.y);
+    const int nwaves = (int)(blockDim.x * blockDim.y) / 64;
+    const int lane = tid & 63, wave = tid >> 6;
+    if (lane == 0) {
+#pragma unroll
+        for (int k = 0; k < NV; k++) sm[wave][k] = v[k];
+    }
+    __syncthreads();
+    if (tid < NV) {
+        double s = 0.0;
+        for (int w = 0; w < nwaves; w++) s += sm[w][tid];
+        out[tid] = s;
+    }
+}
+
 template <typename S>
 static Pops<S> pops_of(const Lattice &L, int copy) {
     Pops<S> P;
@@ -363,6 +391,8 @@ struct Geo2 {
     int xcd_group;               // workgroups per XCD in one round (CUs / 8)
     void *faces;   // FACES: face images of the first step (k_pack_faces layout)
     void *faces2;  // FACES, three-step pass: face images of the second step
+    double *red;   // RED: [2][gridDim.x][RED_NV] partial sums (diagno, dissipation)
+    double svisc;  // RED: the dissipation's viscosity
 };
 
 static Geo2 make_geo2(const Lattice &L, int tx, int ty, int zc, int *nblocks) {
@@ -384,6 +414,8 @@ static Geo2 make_geo2(const Lattice &L, int tx, int ty, int zc, int *nblocks) {
     g.xcd_group = L.ncu >= 8 ? L.ncu / 8 : 1;
     g.faces = nullptr;
     g.faces2 = nullptr;
+    g.red = nullptr;
+    g.svisc = 0.0;
     *nblocks = g.tiles_x * g.tiles_y * ((g.zhi - g.zlo + g.zc - 1) / g.zc);
     return g;
 }
@@ -799,10 +831,21 @@ inline int three_step_zc(const Lattice &L, int tx, int ty, int zc_max) {
 // step's to g.faces2 (the stale halo images the reference's dissipation reads,
 // BGK_FLAG_REFCOMPAT_HALOS); that variant carries one load register set, the
 // face writes' registers would not fit beside the second.
-template <typename S, typename C, int TX, int TY, bool SH, bool SLAB = false, bool FACES = false>
+// RED (periodic lattice): two time steps and the diagnostics of their
+// result in one pass (the reference's loop body at a diagnostic step,
+// SRC/bgk3d_HIT.F90:127-146): the owners store S2 and sum its at-site
+// moments (SRC/diagno.F90:96-111), and the third stage, instead of a
+// collision, sums the dissipation of S2's pulled populations
+// (SRC/dissipation.F90:97-249); per workgroup one slot of diagno sums
+// {rho, ux, uy, uz, |u|^2/2, uz at (nx-1, ny-1, nz-1)} and one of
+// {2 svisc P, |u|^2/2} in g.red, in FP64 as the reduction kernels sum.  One
+// load register set, as FACES.
+template <typename S, typename C, int TX, int TY, bool SH, bool SLAB = false, bool FACES = false,
+          bool RED = false>
 __global__ __launch_bounds__(collide3_threads(TX, TY), 1) void k_collide3(Pops<S> src,
                                                                           Pops<S> dst, Geo2 g,
                                                                           C omega, C cte1) {
+    static_assert(!(RED && (SLAB || FACES)), "RED runs on the periodic lattice alone");
     constexpr int EX = TX + 4, EY = TY + 4;
     constexpr int NMAIN = TX * EY;
     static_assert(4 * EY <= 64, "the edge wave covers four grown columns");
@@ -878,6 +921,29 @@ __global__ __launch_bounds__(collide3_threads(TX, TY), 1) void k_collide3(Pops<S
     S r1O_1 = S{}, r1O_2 = S{}, r1R_1 = S{};  // S1 of this column at planes z-1, z-2
     S r2O_1 = S{}, r2O_2 = S{}, r2R_1 = S{};  // S2 of this column, one plane behind
     const int nit = zb - za + 4;              // iterations z = za-2 .. zb+1
+    // the owner's site of plane p to the destination copy
+    auto store_site = [&](const S(&v)[19], int p) {
+        const ptrdiff_t zo = (ptrdiff_t)(p + zoff) * plane;
+        if constexpr (sizeof(S) == 2) {
+            // global stores from per-population scalar bases: a value in
+            // the high half of a packed pair stores with the d16_hi form
+            // (the offset opaque per plane, so that it stays the 32-bit
+            // offset of the scalar-base store form, see two_step_pass)
+            uint32_t c = X[1] + Y[1];
+            asm volatile("" : "+v"(c));
+#pragma unroll
+            for (int q = 0; q < 19; q++) st_at(plane_ptr(dst.p[q], zo), c, v[q]);
+        } else {
+            const __amdgpu_buffer_rsrc_t R = raw_rsrc(dst.p[0] + zo);
+#pragma unroll
+            for (int q = 0; q < 19; q++)
+                buf_st<S>(v[q], R, X[1] + Y[1], (uint32_t)q * g.pstride);
+        }
+    };
+    // RED: this thread's sums (diagno's five, dissipation's two) and uz of
+    // the last site
+    double rd[5] = {0.0, 0.0, 0.0, 0.0, 0.0}, rs[2] = {0.0, 0.0};
+    C ulast = (C)0;
     auto iteration = [&](int it, S(&cur)[19], S(&nxt)[19]) {
         const int z = za - 2 + it;
         const int pb = pbase(2) + 4 * (it & 1);  // slot of S(z-2), then of S(z)
@@ -911,6 +977,22 @@ __global__ __launch_bounds__(collide3_threads(TX, TY), 1) void k_collide3(Pops<S
                 if (pgroup<1>(q) == 0) l2[pbase(0) + pslot<1>(q)][ey - 1][ex - 1] = s2[q];
             if (FACES && owner && z - 1 >= za && z - 1 < zb)
                 write_faces<S, true>(g.faces2, g, s2, gx, gy, wrapz(z - 1));
+            if constexpr (RED) {
+                if (owner && z - 1 >= za && z - 1 < zb) {
+                    store_site(s2, z - 1);
+                    C f2s[19], rho, ux, uy, uz;
+#pragma unroll
+                    for (int q = 0; q < 19; q++) f2s[q] = Storage<S>::template load<C>(s2[q]);
+                    moments_site<C>(f2s, cte1, rho, ux, uy, uz);
+                    const C sk = (C)0.5 * (ux * ux + uy * uy + uz * uz);
+                    rd[0] += (double)rho;
+                    rd[1] += (double)ux;
+                    rd[2] += (double)uy;
+                    rd[3] += (double)uz;
+                    rd[4] += (double)sk;
+                    if (gx == g.nx - 1 && gy == g.ny - 1 && z - 1 == g.nz - 1) ulast = uz;
+                }
+            }
         }
         lds_barrier();
         if (active) {
@@ -925,29 +1007,22 @@ __global__ __launch_bounds__(collide3_threads(TX, TY), 1) void k_collide3(Pops<S
                 write_faces<S, true>(g.faces, g, s1, gx, gy, wrapz(z));
         }
         if (owner && it >= 4) {  // S3 at plane z-2
-            C f3[19], b[19], ux, uy, uz;
+            C f3[19];
 #pragma unroll
             for (int q = 0; q < 19; q++)
                 f3[q] = Storage<S>::template load<C>(
                     pull(l2, q, ey - 1, ex - 1, pb, s2[QN], r2O_2, r2R_1));
-            collide_site<C, SH>(f3, b, omega, cte1, ux, uy, uz);
-            S s3[19];
-            store_all<S, C>(b, s3);
-            const ptrdiff_t zo = (ptrdiff_t)(wrapz(z - 2) + zoff) * plane;
-            if constexpr (sizeof(S) == 2) {
-                // global stores from per-population scalar bases: a value in
-                // the high half of a packed pair stores with the d16_hi form
-                // (the offset opaque per plane, so that it stays the 32-bit
-                // offset of the scalar-base store form, see two_step_pass)
-                uint32_t c = X[1] + Y[1];
-                asm volatile("" : "+v"(c));
-#pragma unroll
-                for (int q = 0; q < 19; q++) st_at(plane_ptr(dst.p[q], zo), c, s3[q]);
+            if constexpr (RED) {  // dissipation of S2 at plane z-2
+                C dsite, tsite;
+                dissipation_site<C>(f3, cte1, (C)g.svisc, dsite, tsite);
+                rs[0] += (double)dsite;
+                rs[1] += (double)tsite;
             } else {
-                const __amdgpu_buffer_rsrc_t R = raw_rsrc(dst.p[0] + zo);
-#pragma unroll
-                for (int q = 0; q < 19; q++)
-                    buf_st<S>(s3[q], R, X[1] + Y[1], (uint32_t)q * g.pstride);
+                C b[19], ux, uy, uz;
+                collide_site<C, SH>(f3, b, omega, cte1, ux, uy, uz);
+                S s3[19];
+                store_all<S, C>(b, s3);
+                store_site(s3, wrapz(z - 2));
             }
         }
         lds_barrier();
@@ -969,7 +1044,7 @@ __global__ __launch_bounds__(collide3_threads(TX, TY), 1) void k_collide3(Pops<S
 
     S ra[19];
     load_plane(za - 2, ra);
-    if constexpr (FACES) {
+    if constexpr (FACES || RED) {
         for (int it = 0; it < nit; it++) iteration(it, ra, ra);
     } else {
         S rb[19];
@@ -979,6 +1054,13 @@ __global__ __launch_bounds__(collide3_threads(TX, TY), 1) void k_collide3(Pops<S
             iteration(it + 1, rb, ra);
         }
         if (it < nit) iteration(it, ra, rb);
+    }
+    if constexpr (RED) {
+        double v[RED_NV] = {rd[0], rd[1], rd[2], rd[3], rd[4], (double)ulast, 0.0, 0.0};
+        block_sum_store<RED_NV>(v, g.red + (size_t)blockIdx.x * RED_NV);
+        __syncthreads();  // block_sum_store's LDS is reused
+        double w[RED_NV] = {rs[0], rs[1], 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+        block_sum_store<RED_NV>(w, g.red + ((size_t)gridDim.x + blockIdx.x) * RED_NV);
     }
 }
 

@@ -77,8 +77,9 @@ struct Timing {
         hipEvent_t a, b;
     };
     std::vector<Rec> recs;
-    double ms[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    long long launches[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    static constexpr int NCLS = 16;  // room for the TCLS_ classes
+    double ms[NCLS] = {};
+    long long launches[NCLS] = {};
 };
 
 }  // namespace bgk
@@ -152,8 +153,11 @@ void lattice_geometry(Lattice &L, int lx, int ly, int lz, int storage, int layou
 // TCLS_PASS2 / TCLS_PASS3: collide launches of two / three time steps
 // (counted with the collide launches in bgk_timing); TCLS_TAPE / TCLS_ADJOINT:
 // the adjoint's taped forward launches and its adjoint-step launches
+// TCLS_PASS2D: passes of two time steps and the diagnostics of their result
+// (counted with the collide launches in bgk_timing)
 enum { TCLS_COLLIDE = 0, TCLS_REDUCE = 1, TCLS_HALO = 2, TCLS_STEP = 3, TCLS_PASS2 = 4,
-       TCLS_PASS3 = 5, TCLS_TAPE = 6, TCLS_ADJOINT = 7, TCLS_N = 8 };
+       TCLS_PASS3 = 5, TCLS_TAPE = 6, TCLS_ADJOINT = 7, TCLS_PASS2D = 8, TCLS_N = 9 };
+static_assert(TCLS_N <= Timing::NCLS, "timing classes");
 hipError_t timing_begin(bgk_ctx *c, int cls, hipStream_t s, hipEvent_t *out);
 hipError_t timing_end(bgk_ctx *c, int cls, hipStream_t s, hipEvent_t begin);
 
@@ -176,6 +180,13 @@ bool three_step_ok(const Lattice &L);
 constexpr int SLAB3_MIN_NZ = 12;
 hipError_t launch_collide3(const Lattice &L, int src, double omega, void *faces1, void *faces2,
                            hipStream_t s);
+// two time steps from copy `src` into copy 1-src and the diagnostics of
+// the result in one pass (periodic layout, where three_step_ok(L) holds):
+// partials holds 2 * 8 * collide3_diag_parts(L) doubles, out 16 (out[0..7]
+// as launch_diagno's, out[8..15] as launch_dissipation's, consistent mode)
+int collide3_diag_parts(const Lattice &L);
+hipError_t launch_collide3_diag(const Lattice &L, int src, double omega, double svisc,
+                                double *partials, double *out, hipStream_t s);
 // diagno and dissipation in one pass: partials holds 2 * 8 * parts doubles,
 // out 16 (out[0..7] as launch_diagno's, out[8..15] as launch_dissipation's)
 hipError_t launch_diag_diss(const Lattice &L, int cur, const void *gcache, double svisc,

@@ -117,34 +117,8 @@ static hipError_t collide3_t(const Lattice &L, int src, double omega, void *face
 
 // --------------------------------------------------------------- reductions
 constexpr int RED_THREADS = 256;
-constexpr int RED_NV = 8;
 constexpr int FIN_THREADS = 1024;              // threads of the final combine
 constexpr long long RED_TARGET_PARTS = 16384;  // partial sums per reduction
-
-// Reduce NV doubles over the block (any size up to 1024 threads, all threads
-// taking part) and write them to out[0..NV).
-template <int NV>
-__device__ __forceinline__ void block_sum_store(double (&v)[NV], double *out) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-#pragma unroll
-        for (int k = 0; k < NV; k++) v[k] += __shfl_xor(v[k], off);
-    }
-    __shared__ double sm[16][NV];
-    const int tid = (int)(threadIdx.x + blockDim.x * threadIdx.y);
-    const int nwaves = (int)(blockDim.x * blockDim.y) / 64;
-    const int lane = tid & 63, wave = tid >> 6;
-    if (lane == 0) {
-#pragma unroll
-        for (int k = 0; k < NV; k++) sm[wave][k] = v[k];
-    }
-    __syncthreads();
-    if (tid < NV) {
-        double s = 0.0;
-        for (int w = 0; w < nwaves; w++) s += sm[w][tid];
-        out[tid] = s;
-    }
-}
 
 __global__ __launch_bounds__(FIN_THREADS) void k_finalize(const double *partials, int nblocks,
                                                           double *out) {
@@ -377,6 +351,48 @@ static hipError_t diag_diss_t(const Lattice &L, int cur, const void *gcache, dou
     if ((e = hipGetLastError()) != hipSuccess) return e;
     hipLaunchKernelGGL(k_finalize, dim3(1), dim3(FIN_THREADS), 0, s, ps, g.nparts, out + RED_NV);
     return hipGetLastError();
+}
+
+// Two time steps from copy `src` into copy 1-src and the diagnostics of the
+// result in one pass (k_collide3 with RED; periodic lattice): partials
+// [2][nblocks][RED_NV], combined into out[0..8) (diagno's sums, as
+// k_diag_diss's first set) and out[8..16) (dissipation's).
+template <typename S, typename C, bool SH>
+static hipError_t collide3_diag_sh(const Lattice &L, int src, double omega, double svisc,
+                                   double *partials, double *out, hipStream_t s) {
+    using T = Tile3<S, C>;
+    const int zc = three_step_zc(L, T::X, T::Y, T::ZC);
+    if (zc <= 0 || L.layout != BGK_LAYOUT_PERIODIC) return hipErrorInvalidValue;
+    int nblocks;
+    Geo2 g = make_geo2(L, T::X, T::Y, zc, &nblocks);
+    g.red = partials;
+    g.svisc = svisc;
+    const C w = (C)omega, cte1 = (C)(L.shifted ? 1.0 : 0.0);
+    hipLaunchKernelGGL((k_collide3<S, C, T::X, T::Y, SH, false, false, true>), dim3(nblocks),
+                       dim3(collide3_threads(T::X, T::Y)), 0, s, pops_of<S>(L, src),
+                       pops_of<S>(L, 1 - src), g, w, cte1);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(FIN_THREADS), 0, s, partials, nblocks, out);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(FIN_THREADS), 0, s,
+                       partials + (size_t)nblocks * RED_NV, nblocks, out + RED_NV);
+    return hipGetLastError();
+}
+template <typename S, typename C>
+static hipError_t collide3_diag_t(const Lattice &L, int src, double omega, double svisc,
+                                  double *partials, double *out, hipStream_t s) {
+    return L.shifted ? collide3_diag_sh<S, C, true>(L, src, omega, svisc, partials, out, s)
+                     : collide3_diag_sh<S, C, false>(L, src, omega, svisc, partials, out, s);
+}
+template <typename S, typename C>
+static int collide3_diag_parts_t(const Lattice &L) {
+    using T = Tile3<S, C>;
+    const int zc = three_step_zc(L, T::X, T::Y, T::ZC);
+    if (zc <= 0) return 0;
+    int nblocks;
+    (void)make_geo2(L, T::X, T::Y, zc, &nblocks);
+    return nblocks;
 }
 
 // ------------------------------------------------------------- checksum
@@ -763,6 +779,20 @@ bool three_step_ok(const Lattice &L) {
 hipError_t launch_collide3(const Lattice &L, int src, double omega, void *faces1, void *faces2,
                            hipStream_t s) {
     BGK_DISPATCH(L.storage, collide3_t, L, src, omega, faces1, faces2, s);
+}
+hipError_t launch_collide3_diag(const Lattice &L, int src, double omega, double svisc,
+                                double *partials, double *out, hipStream_t s) {
+    BGK_DISPATCH(L.storage, collide3_diag_t, L, src, omega, svisc, partials, out, s);
+}
+int collide3_diag_parts(const Lattice &L) {
+    switch (L.storage) {
+    case BGK_STORE_F64: return collide3_diag_parts_t<double, double>(L);
+    case BGK_STORE_F32: return collide3_diag_parts_t<float, float>(L);
+    case BGK_STORE_F32_MIXED: return collide3_diag_parts_t<float, double>(L);
+    case BGK_STORE_F16: return collide3_diag_parts_t<half_t, float>(L);
+    case BGK_STORE_BF16: return collide3_diag_parts_t<bf16_t, float>(L);
+    default: return 0;
+    }
 }
 hipError_t launch_collide2(const Lattice &L, int src, double omega, void *faces,
                            hipStream_t s) {

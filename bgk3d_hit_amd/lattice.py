@@ -175,6 +175,14 @@ class Lattice:
                                           else L.DIAG_CONSISTENT, _dptr(out)))
         return out[:5].copy(), out[5:].copy()
 
+    def step_diagnostics(self, nsteps, svisc):
+        """step(nsteps), then diagnostics(svisc) of the state reached
+        (consistent mode); the last two steps and the reductions share one
+        pass where the lattice allows it.  Returns (5 values, 2 values)."""
+        out = np.zeros(7)
+        L.check(self._lib.bgk_step_diagnostics(self._h, int(nsteps), float(svisc), _dptr(out)))
+        return out[:5].copy(), out[5:].copy()
+
     def get_box(self, x0, x1, y0, y1, z0, z1):
         """Populations of a sub-box, shape (19, z1-z0, y1-y0, x1-x0)."""
         out = np.empty((19, z1 - z0, y1 - y0, x1 - x0))
@@ -203,6 +211,13 @@ class Lattice:
         t = np.zeros(6)
         L.check(self._lib.bgk_timing_passes(self._h, _dptr(t), 1 if reset else 0))
         return {k + 1: (t[2 * k], int(t[2 * k + 1])) for k in range(3)}
+
+    def timing_fused(self, reset=True):
+        """Kernel time of the two-steps-plus-diagnostics passes of
+        step_diagnostics (BGK_FLAG_TIMING): (ms, launches)."""
+        t = np.zeros(2)
+        L.check(self._lib.bgk_timing_fused(self._h, _dptr(t), 1 if reset else 0))
+        return t[0], int(t[1])
 
     def timing_adjoint(self, reset=True):
         """Kernel time of the adjoint's taped forward passes and adjoint-step

@@ -170,6 +170,19 @@ int bgk_dissipation(bgk_ctx *ctx, double svisc, int mode, double out[2]);
  * reference routine does both.                                              */
 int bgk_diagnostics(bgk_ctx *ctx, double svisc, int mode, double out[7]);
 
+/* nsteps time steps (as bgk_step), then bgk_diagnostics of the state they
+ * leave in consistent mode (out as bgk_diagnostics').  Replaces the
+ * reference's loop body at a diagnostic step (SRC/bgk3d_HIT.F90:127-146:
+ * col_MC and boundaries, then diagnostic -> diagno + dissipation,
+ * SRC/diagnostic.F90:47-49).  On the periodic lattice without halo images
+ * (BGK_FLAG_REFCOMPAT_HALOS clear), where three-step passes run, and for
+ * nsteps = 3k + 2 (k >= 0) or 3k + 1 (k >= 1), the last two steps and the
+ * reductions run as one pass over the populations; otherwise it is
+ * bgk_step + bgk_diagnostics.  The populations are bit-identical to
+ * bgk_step's; the sums are the same per-site values summed in another
+ * order (equal to bgk_diagnostics' to rounding of the FP64 sums).           */
+int bgk_step_diagnostics(bgk_ctx *ctx, int nsteps, double svisc, double out[7]);
+
 /* Allocation-free variant of bgk_diagnostics.  No counterpart in the
  * reference: it is for callers that cannot allocate device memory (code
  * under HIP graph capture, tight serving loops, harnesses that forbid
@@ -230,6 +243,12 @@ int bgk_timing(bgk_ctx *ctx, double t[8], int reset);
  * (k_collide3).  reset != 0 clears every counter (as bgk_timing's does).
  * No counterpart in the reference (its timers are per section).             */
 int bgk_timing_passes(bgk_ctx *ctx, double t[6], int reset);
+
+/* With BGK_FLAG_TIMING: t[0], t[1] = ms and launches of the passes of two
+ * time steps and the diagnostics of their result (bgk_step_diagnostics),
+ * also counted in bgk_timing's collide launches t[0..1].  reset != 0 clears
+ * every counter.  No counterpart in the reference.                          */
+int bgk_timing_fused(bgk_ctx *ctx, double t[2], int reset);
 
 /* With BGK_FLAG_TIMING: the kernel time of the discrete adjoint's launches
  * (bgk_adjoint_ke): t[0], t[1] = ms and launches of its taped forward passes
