@@ -149,6 +149,18 @@ module bgk_capi
        real(c_double), intent(out) :: out(7)
      end function bgk_diagnostics
 
+     ! nsteps time steps, then bgk_diagnostics (consistent mode) of the state
+     ! reached; the last two steps and the sums share one pass where the
+     ! lattice allows it
+     integer(c_int) function bgk_step_diagnostics(ctx, nsteps, svisc, out) &
+          bind(c, name='bgk_step_diagnostics')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: nsteps
+       real(c_double), value :: svisc
+       real(c_double), intent(out) :: out(7)
+     end function bgk_step_diagnostics
+
      integer(c_int) function bgk_get_box(ctx, x0, x1, y0, y1, z0, z1, out) &
           bind(c, name='bgk_get_box')
        import :: c_int, c_ptr, c_double
