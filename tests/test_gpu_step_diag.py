@@ -150,3 +150,31 @@ def test_512cubed_window(gpu, kind):
     finally:
         b.close()
     check_sums(d_f, s_f, d_s, s_s)
+
+
+def test_argument_errors(gpu):
+    """Negative step counts and a null output are refused before any work,
+    and the state is left as it was."""
+    import ctypes
+
+    from bgk3d_hit_amd import _lib as L
+
+    shape = (48, 40, 36)
+    a, _ = pair(shape, "f32")
+    _.close()
+    try:
+        h0 = a.checksum()
+        out = (ctypes.c_double * 7)()
+        assert a._lib.bgk_step_diagnostics(a._h, -1, 0.01, out) == L.EINVAL
+        assert "nsteps" in L.last_error()
+        assert a._lib.bgk_step_diagnostics(a._h, 5, 0.01, None) == L.EINVAL
+        assert a.checksum() == h0
+        with pytest.raises(L.BGKError):
+            a.step_diagnostics(-3, 0.01)
+        # zero steps: the diagnostics of the current state
+        d0, s0 = a.step_diagnostics(0, 0.01)
+        d1, s1 = a.diagnostics(0.01)
+        assert np.array_equal(d0, d1) and np.array_equal(s0, s1)
+        assert a.checksum() == h0
+    finally:
+        a.close()
