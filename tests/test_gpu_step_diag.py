@@ -178,3 +178,31 @@ def test_argument_errors(gpu):
         assert a.checksum() == h0
     finally:
         a.close()
+
+
+def test_512cubed_2000_steps_diagnostics_every_50(gpu):
+    """The full headline run (512^3 FP32, omega 1.95, 2000 steps, the
+    diagnostics every 50 steps): 40 fused calls against 40 x (bgk_step(50) +
+    bgk_diagnostics), every diagnostic compared, the final states equal."""
+    shape = (512, 512, 512)
+    nu = bgk.svisc_from_omega(1.95)
+    say("512^3 FP32 2000 steps, diagnostics every 50")
+    a = bgk.Lattice(*shape, omega=1.95, storage="f32", timing=True)
+    b = bgk.Lattice(*shape, omega=1.95, storage="f32")
+    try:
+        a.init_hit(4.0, 0.02, 1234)
+        b.init_hit(4.0, 0.02, 1234)
+        ke0 = a.diagno()[4]
+        for blk in range(40):
+            d_f, s_f = a.step_diagnostics(50, nu)
+            b.step(50)
+            d_s, s_s = b.diagnostics(nu)
+            check_sums(d_f, s_f, d_s, s_s)
+        assert a.checksum() == b.checksum()
+        assert a.timing_fused()[1] == 40
+        # decaying turbulence: energy falls, mass stays
+        assert d_f[4] < ke0
+        assert abs(d_f[0] - 1.0) < 1e-6
+    finally:
+        a.close()
+        b.close()
