@@ -46,7 +46,7 @@ def check_sums(d_f, s_f, d_s, s_s):
 
 
 @pytest.mark.parametrize("kind", ["f64", "f32", "f32mix", "f16", "bf16"])
-@pytest.mark.parametrize("shape", [(70, 36, 44), (64, 40, 67)])
+@pytest.mark.parametrize("shape", [(70, 36, 44), (64, 40, 67), (40, 36, 4), (33, 17, 5)])
 def test_fused_equals_step_then_diagnostics(gpu, kind, shape):
     nu = bgk.svisc_from_omega(OMEGA)
     a, b = pair(shape, kind)
