@@ -206,3 +206,37 @@ def test_512cubed_2000_steps_diagnostics_every_50(gpu):
     finally:
         a.close()
         b.close()
+
+
+@pytest.mark.parametrize("kind", ["f64", "f32", "f16"])
+def test_unshifted_storage(gpu, kind):
+    """Populations stored as f_i (BGK_FLAG_NOSHIFT): the fused pass's
+    unshifted instantiation against bgk_step + bgk_diagnostics and the oracle."""
+    import oracle
+
+    shape = (44, 30, 26)
+    nu = bgk.svisc_from_omega(OMEGA)
+    say("step_diagnostics unshifted", kind, shape)
+    u = bgk.hit_velocity(*shape, 3.0, 0.04, 9)
+    a = bgk.Lattice(*shape, omega=OMEGA, storage=kind, shifted=False, timing=True,
+                    two_step="always")
+    b = bgk.Lattice(*shape, omega=OMEGA, storage=kind, shifted=False, two_step="always")
+    o = oracle.OracleLattice(*shape, omega=OMEGA, kind=kind, shifted=False)
+    try:
+        for lat in (a, b, o):
+            lat.init_equilibrium(u)
+        d_f, s_f = a.step_diagnostics(8, nu)
+        b.step(8)
+        d_s, s_s = b.diagnostics(nu)
+        o.step(8)
+        assert a.checksum() == b.checksum()
+        assert np.array_equal(a.get_populations(), o.get_populations())
+        check_sums(d_f, s_f, d_s, s_s)
+        assert np.abs(d_f - o.diagno()).max() <= 1e-12
+        o.bcond_step3()
+        so = o.dissipation(nu)
+        assert np.all(np.abs(s_f - so) <= 1e-10 * np.abs(so))
+        assert a.timing_fused()[1] == 1
+    finally:
+        a.close()
+        b.close()
