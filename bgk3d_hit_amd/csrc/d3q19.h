@@ -146,6 +146,30 @@ struct K {
     static constexpr C p2 = (C)(1.0 / (3.0 * 3.0 * (1.0 + 3.0)));
 };
 
+// uno / x, the reference's rhoinv (SRC/col_MC.F90:135), rounded to nearest
+// even as the IEEE quotient.  FP32: the hardware reciprocal estimate
+// (v_rcp_f32, within one ulp) and one Newton-Raphson correction with fused
+// multiply-adds, e = 1 - x y, y + e y (Markstein's correction); it equals
+// the IEEE quotient 1.0f / x for every x in [2^-125, 2^126), all
+// 2 105 540 608 of them compared on gfx950 (bin/recip_check, run by the GPU
+// tests), in 3 instructions where the quotient takes 10 (two scale steps,
+// the estimate, five fused steps, the fix-ups).  Any other x (zero,
+// denormal, negative, huge, Inf, NaN: a lattice whose density left that
+// range has diverged) takes the quotient itself.  FP64: the quotient.
+template <typename C>
+__device__ __forceinline__ C recip_rn(C x) {
+    return K<C>::uno / x;
+}
+template <>
+__device__ __forceinline__ float recip_rn<float>(float x) {
+    if (x >= 0x1p-125f && x < 0x1p126f) {
+        const float y = __builtin_amdgcn_rcpf(x);
+        const float e = __builtin_fmaf(-x, y, 1.0f);
+        return __builtin_fmaf(e, y, y);
+    }
+    return 1.0f / x;
+}
+
 // cte0 + y for the quadratic terms of collide_site.  SH (shifted storage,
 // cte1 = 1, cte0 = +0): +0 + y equals y bit for bit unless y is -0, and no y
 // passed here is -0: y = qf * (a - vsq) with a = (tre * v) * v or tre * v*v,
@@ -183,7 +207,7 @@ __device__ __forceinline__ void collide_site(const C (&x)[19], C (&b)[19], C ome
 
     const C rho = x01 + x02 + x03 + x04 + x05 + x06 + x07 + x08 + x09 + x10 + x11 + x12 + x13 +
                   x14 + x15 + x16 + x17 + x18 + x19 + cte1;
-    const C rhoinv = k::uno / rho;
+    const C rhoinv = recip_rn<C>(rho);
     C vx = (x01 + x02 + x03 + x04 + x05 - x10 - x11 - x12 - x13 - x14) * rhoinv;
     C vy = (x03 + x07 + x08 + x09 + x12 - x01 - x10 - x16 - x17 - x18) * rhoinv;
     C vz = (x04 + x06 + x07 + x13 + x18 - x02 - x09 - x11 - x15 - x16) * rhoinv;
@@ -272,7 +296,7 @@ __device__ __forceinline__ void moments_site(const C (&x)[19], C cte1, C &rho, C
     const C x14 = x[13], x15 = x[14], x16 = x[15], x17 = x[16], x18 = x[17], x19 = x[18];
     rho = x01 + x02 + x03 + x04 + x05 + x06 + x07 + x08 + x09 + x10 + x11 + x12 + x13 + x14 +
           x15 + x16 + x17 + x18 + x19 + cte1;
-    const C rhoinv = K<C>::uno / rho;
+    const C rhoinv = recip_rn<C>(rho);
     ux = (x01 + x02 + x03 + x04 + x05 - x10 - x11 - x12 - x13 - x14) * rhoinv;
     uy = (x03 + x07 + x08 + x09 + x12 - x01 - x10 - x16 - x17 - x18) * rhoinv;
     uz = (x04 + x06 + x07 + x13 + x18 - x02 - x09 - x11 - x15 - x16) * rhoinv;
@@ -290,7 +314,7 @@ __device__ __forceinline__ void dissipation_site(const C (&x)[19], C cte1, C svi
     const C x14 = x[13], x15 = x[14], x16 = x[15], x17 = x[16], x18 = x[17], x19 = x[18];
     const C rho = x01 + x02 + x03 + x04 + x05 + x06 + x07 + x08 + x09 + x10 + x11 + x12 + x13 +
                   x14 + x15 + x16 + x17 + x18 + x19 + cte1;
-    const C rhoinv = k::uno / rho;
+    const C rhoinv = recip_rn<C>(rho);
     C vx = (x01 + x02 + x03 + x04 + x05 - x10 - x11 - x12 - x13 - x14) * rhoinv;
     C vy = (x03 + x07 + x08 + x09 + x12 - x01 - x10 - x16 - x17 - x18) * rhoinv;
     C vz = (x04 + x06 + x07 + x13 + x18 - x02 - x09 - x11 - x15 - x16) * rhoinv;
