@@ -570,8 +570,12 @@ int adjoint_t(bgk_ctx *c, const double *u0, int T, int K, int levels, double *J,
     hipStream_t s1 = c->s1;
 
     // ---- forward (stream s0): f^0 from u0, coarse checkpoints every K2
-    // steps; the last segment's tape is recorded on the way
-    const int last_s0 = nseg > 0 ? (nseg - 1) * K : 0;
+    // steps; the tapes of the last two segments (the backward's first two)
+    // are recorded on the way, the last into tape[0] and the one before into
+    // tape[1], so that the backward re-tapes from its third segment on
+    const int ntaped = nseg > 1 ? 2 : (nseg > 0 ? 1 : 0);
+    const int taped_s0 = nseg > 0 ? (nseg - ntaped) * K : 0;  // first taped step
+    const int last_s0 = nseg > 0 ? (nseg - 1) * K : 0;        // the last segment
     auto t0 = std::chrono::steady_clock::now();
     BGK_HIP(launch_init_equilibrium(L, c->cur, u0, N, 0, N, c->s0));
     // Checkpoint i is kept in place: the buffer holding S_{i K2} becomes
@@ -581,9 +585,12 @@ int adjoint_t(bgk_ctx *c, const double *u0, int T, int K, int levels, double *J,
     for (int t = 0; t < T;) {
         // up to the next checkpoint; the last segment taped (fused passes
         // where they pay)
-        const int n = std::min(K2 - t % K2, (t >= last_s0 ? T : last_s0) - t);
-        const bool taped = t >= last_s0;
-        C *tp = taped ? tape[0] + (size_t)(t - last_s0) * 3 * N : nullptr;
+        const int stop = t < taped_s0 ? taped_s0 : (t < last_s0 ? last_s0 : T);
+        const int n = std::min(K2 - t % K2, stop - t);
+        const bool taped = t >= taped_s0;
+        C *tp = !taped          ? nullptr
+                : t >= last_s0 ? tape[0] + (size_t)(t - last_s0) * 3 * N
+                               : tape[1] + (size_t)(t - taped_s0) * 3 * N;
         auto run = [&](int m) {
             return taped ? run_taped<S, C>(c, tp, m, c->s0) : step_periodic(c, m, false, c->s0);
         };
@@ -607,12 +614,13 @@ int adjoint_t(bgk_ctx *c, const double *u0, int T, int K, int levels, double *J,
     BGK_HIP(hipStreamSynchronize(c->s0));
     const double fwd_ms = ms_since(t0);
 
-    // ---- backward.  The segments in backward order; segment 0 (the last)
-    // was taped by the forward into tape[0], segment k uses tape[k % 2].
-    // The sweeps run on s0; everything on the lattice state (fine checkpoints
-    // of a coarse segment, restores, re-taping) runs on s1, so the re-tape of
-    // segment k+1 overlaps the sweep of segment k.  Cross-stream order:
-    //   sweep k waits for the re-tape of segment k (ev_tape);
+    // ---- backward.  The segments in backward order; segments 0 (the last)
+    // and 1 were taped by the forward into tape[0] and tape[1], segment k
+    // uses tape[k % 2].  The sweeps run on s0; everything on the lattice
+    // state (fine checkpoints of a coarse segment, restores, re-taping) runs
+    // on s1, so the re-tape of segment k+1 overlaps the sweep of segment k
+    // (k >= 1; the sweep of segment 0 runs alone).  Cross-stream order:
+    //   sweep k (k >= 2) waits for the re-tape of segment k (ev_tape);
     //   the re-tape of segment k+1 into tape[(k+1) % 2] waits for the sweep
     //   of segment k-1, which read that buffer (ev_sweep).
     t0 = std::chrono::steady_clock::now();
@@ -658,9 +666,8 @@ int adjoint_t(bgk_ctx *c, const double *u0, int T, int K, int levels, double *J,
     };
     C *lin = lamA, *lout = lamB;
     const int nsegs = (int)segs.size();
-    if (nsegs > 1 && (r = retape(1))) return r;
     for (int k = 0; k < nsegs; k++) {
-        if (k >= 1) BGK_HIP(hipStreamWaitEvent(c->s0, ev_tape.e[k % 2], 0));
+        if (k >= 2) BGK_HIP(hipStreamWaitEvent(c->s0, ev_tape.e[k % 2], 0));
         if ((r = adjoint_sweep<C>(c, lin, lout, tape[k % 2], segs[k].s1 - segs[k].s0)))
             return r;
         BGK_HIP(hipEventRecord(ev_sweep.e[k % 2], c->s0));

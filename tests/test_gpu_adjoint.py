@@ -77,10 +77,13 @@ def test_adjoint_checkpoint_levels_agree_bitwise(gpu):
     for J, g, st in res[1:]:
         assert J == res[0][0]
         assert np.array_equal(g, res[0][1])
-    # one level: every segment but the last (taped by the forward) is re-run
-    assert res[1][2]["recomputed_steps"] == ((T + 3) // 4 - 1) * 4
-    # two levels: 3 coarse segments of 12 steps; fine checkpoints re-run too
-    assert res[2][2]["recomputed_steps"] == 44
+    # one level: every segment but the last two (taped by the forward) is
+    # re-run: segments [0,4) .. [20,24) of [0,30)
+    assert res[1][2]["recomputed_steps"] == ((T + 3) // 4 - 2) * 4
+    # two levels: 3 coarse segments of 12 steps; the same 24 re-taped steps
+    # and the fine checkpoints of the first two coarse segments re-run
+    # (8 steps each)
+    assert res[2][2]["recomputed_steps"] == 40
 
 
 def test_adjoint_central_differences_fp64(gpu):
