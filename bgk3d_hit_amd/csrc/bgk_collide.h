@@ -797,6 +797,22 @@ struct Tile3 {
 // one wave for the four grown columns (4 (TY+4) <= 64)
 constexpr int collide3_threads(int tx, int ty) { return tx * (ty + 4) + 64; }
 
+// Where a three-step pass issues the loads of the next plane: populations
+// [0, A) after the first stage's collision, [A, B) after the second's,
+// [B, 19) after the third's (by the compute kind's size).  Measured at 512^3,
+// ms per pass against all 19 loads at the top of the iteration: FP32 5.09-5.33
+// -> 4.56-4.68 (split 10/9; 12/7 4.64, 8/11 4.61-4.63, 10/5/4 4.62-4.66,
+// 7/6/6 4.75-4.88), FP16 4.43 -> 4.12-4.24, BF16 4.48-4.53 -> 4.03-4.15,
+// FP64 10.94 -> 9.80-9.82 (7/6/6; 10/9 10.18-10.51).
+template <int CSIZE>
+struct Load3 {
+    static constexpr int A = 10, B = 19;
+};
+template <>
+struct Load3<8> {
+    static constexpr int A = 7, B = 13;
+};
+
 // Planes per workgroup of the three-step pass: as two_step_zc, with zc + 4
 // plane iterations per chunk.  0 when the launch would not reach half of the
 // CUs (unless L.two_step == 2) or the lattice is too thin in z (periodic:
@@ -895,7 +911,9 @@ __global__ __launch_bounds__(collide3_threads(TX, TY), 1) void k_collide3(Pops<S
     auto wrapz = [&](int z) {
         return SLAB ? z : (z < 0 ? z + g.nz : (z >= g.nz ? z - g.nz : z));
     };
-    auto load_plane = [&](int z, S(&raw)[19]) {
+    // the loads of populations [QA, QB) of plane z
+    auto load_part = [&](int z, S(&raw)[19], auto qa, auto qb) {
+        constexpr int QA = decltype(qa)::value, QB = decltype(qb)::value;
         const int p = wrapz(z);
         const int zm = SLAB ? p - 1 : ((p == 0) ? g.nz - 1 : p - 1);
         const int zp = SLAB ? p + 1 : ((p == g.nz - 1) ? 0 : p + 1);
@@ -904,9 +922,19 @@ __global__ __launch_bounds__(collide3_threads(TX, TY), 1) void k_collide3(Pops<S
                                              raw_rsrc(b0 + (ptrdiff_t)(p + zoff) * plane),
                                              raw_rsrc(b0 + (ptrdiff_t)(zp + zoff) * plane)};
 #pragma unroll
-        for (int q = 0; q < 19; q++)
+        for (int q = QA; q < QB; q++)
             raw[q] = buf_ld<S>(R[1 - cz(q)], X[1 - cx(q)] + Y[1 - cy(q)], (uint32_t)q * g.pstride);
     };
+    // The next plane's loads go out in parts after the stages' collisions
+    // (Load3): issued all at once at the top of the iteration, the 285
+    // loads of a workgroup queue behind each other and the youngest waves
+    // of each SIMD wait thousands of cycles to issue theirs before they can
+    // start the first stage.
+    using LP = Load3<sizeof(C)>;
+    std::integral_constant<int, 0> q0;
+    std::integral_constant<int, LP::A> qA;
+    std::integral_constant<int, LP::B> qB;
+    std::integral_constant<int, 19> q19;
     // pulled value of population q for the site (sy, sx) of a state whose
     // x/y-moving populations are in `lds` (newest plane group 0, slot pb of
     // the plane two before); the z-only ones from the column's registers
@@ -954,7 +982,7 @@ __global__ __launch_bounds__(collide3_threads(TX, TY), 1) void k_collide3(Pops<S
             f[q] = Storage<S>::template load<C>(cur[q]);
             asm volatile("" : "+v"(f[q]));  // see two_step_pass
         }
-        if (it + 1 < nit) load_plane(z + 1, nxt);
+        const bool more = it + 1 < nit;
         if (active) {  // S1 at plane z
             C b[19], ux, uy, uz;
             collide_site<C, SH>(f, b, omega, cte1, ux, uy, uz);
@@ -963,6 +991,7 @@ __global__ __launch_bounds__(collide3_threads(TX, TY), 1) void k_collide3(Pops<S
             for (int q = 0; q < 19; q++)
                 if (pgroup<1>(q) == 0) l1[pbase(0) + pslot<1>(q)][ey][ex] = s1[q];
         }
+        if (more) load_part(z + 1, nxt, q0, qA);
         lds_barrier();
         const bool do2 = mid && it >= 2;
         if (do2) {  // S2 at plane z-1
@@ -994,6 +1023,7 @@ __global__ __launch_bounds__(collide3_threads(TX, TY), 1) void k_collide3(Pops<S
                 }
             }
         }
+        if (more) load_part(z + 1, nxt, qA, qB);
         lds_barrier();
         if (active) {
 #pragma unroll
@@ -1025,6 +1055,7 @@ __global__ __launch_bounds__(collide3_threads(TX, TY), 1) void k_collide3(Pops<S
                 store_site(s3, wrapz(z - 2));
             }
         }
+        if (more) load_part(z + 1, nxt, qB, q19);
         lds_barrier();
         if (do2) {
 #pragma unroll
@@ -1043,7 +1074,7 @@ __global__ __launch_bounds__(collide3_threads(TX, TY), 1) void k_collide3(Pops<S
     };
 
     S ra[19];
-    load_plane(za - 2, ra);
+    load_part(za - 2, ra, q0, q19);
     if constexpr (FACES || RED) {
         for (int it = 0; it < nit; it++) iteration(it, ra, ra);
     } else {
