@@ -63,48 +63,120 @@ __global__ __launch_bounds__(256) void k_lambda_final(Pops<S> P, SiteMap m, C *M
     }
 }
 
-// a + c v for c in {-1, 0, +1} known at compile time: the c = 0 term is
-// left out and c = +-1 is an add or subtract (without fast-math the
-// compiler must keep a multiplication by 0 and the addition of its signed
-// zero; leaving them out changes at most the sign of a zero sum)
-template <typename C>
-__device__ __forceinline__ C add_c(C a, int c, C v) {
-    return c == 0 ? a : (c > 0 ? a + v : a - v);
+// Index of the population with velocity s e_k (axis), and of the one with
+// components sk on axis k and sl on axis l (diagonal), in the velocity table.
+__host__ __device__ constexpr int comp_c(int q, int k) {
+    return k == 0 ? cx(q) : (k == 1 ? cy(q) : cz(q));
 }
-// c_q . u over the nonzero components of c_q
-template <typename C>
-__device__ __forceinline__ C dot_c(int q, C ux, C uy, C uz) {
-    const int n = (cx(q) != 0) + (cy(q) != 0) + (cz(q) != 0);
-    if (n == 0) return (C)0;
-    C r = cx(q) != 0 ? (cx(q) > 0 ? ux : -ux) : (cy(q) != 0 ? (cy(q) > 0 ? uy : -uy)
-                                                            : (cz(q) > 0 ? uz : -uz));
-    if (cx(q) != 0 && cy(q) != 0) r = add_c(r, cy(q), uy);
-    if ((cx(q) != 0 || cy(q) != 0) && cz(q) != 0) r = add_c(r, cz(q), uz);
+__host__ __device__ constexpr int axis_q(int k, int s) {
+    int r = -1;
+    for (int q = 0; q < 19; q++)
+        if (comp_c(q, k) == s && comp_c(q, (k + 1) % 3) == 0 && comp_c(q, (k + 2) % 3) == 0)
+            r = q;
+    return r;
+}
+__host__ __device__ constexpr int diag_q(int k, int l, int sk, int sl) {
+    int r = -1;
+    for (int q = 0; q < 19; q++)
+        if (comp_c(q, k) == sk && comp_c(q, l) == sl && comp_c(q, 3 - k - l) == 0) r = q;
+    return r;
+}
+__host__ __device__ constexpr int rest_q() {
+    int r = -1;
+    for (int q = 0; q < 19; q++)
+        if (cx(q) == 0 && cy(q) == 0 && cz(q) == 0) r = q;
     return r;
 }
 
-// mu_k = (1 - omega) lambda_k + omega [A + (B - 3 S u) . c_k] at one site
+// mu_k = (1 - omega) lambda_k + omega [A + G . c_k] at one site, through the
+// weighted moments of lambda: m0 = sum w lambda, m1 = sum w lambda c,
+// P = sum w lambda c c^T, so that
+//   A = sum_i w_i lambda_i (1 - 9/2 (c_i.u)^2 + 3/2 |u|^2)
+//     = m0 (1 + 3/2 |u|^2) - 9/2 u.P u,
+//   G = sum_i w_i lambda_i (3 + 9 c_i.u) c_i - 3 S u = 3 (m1 - m0 u) + 9 P u.
+// The moments come from unweighted sums over the axis populations and the
+// four diagonals of each coordinate plane, each weight applied once (the
+// adjoint has no counterpart in the reference, so its arithmetic is free to
+// take this form, with fused multiply-adds): ~130 VALU operations per site
+// against ~330 for the per-population sums.
+// fused multiply-add in the compute kind
+__device__ __forceinline__ float fma_k(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double fma_k(double a, double b, double c) { return __builtin_fma(a, b, c); }
+
 template <typename C>
 __device__ __forceinline__ void adjoint_site(const C (&lam)[19], C ux, C uy, C uz, C omega,
                                              C (&mu)[19]) {
-    const C usq = ux * ux + uy * uy + uz * uz;
-    C A = 0, Bx = 0, By = 0, Bz = 0, Sw = 0;
+    const C w0 = (C)(1.0 / 3.0), w1 = (C)(1.0 / 18.0), w2 = (C)(1.0 / 36.0);
+    const C u[3] = {ux, uy, uz};
+    // axis populations: c_k lambda and c_k^2 lambda summed along each axis
+    C a1[3], a2[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        const C p = lam[axis_q(k, 1)], m = lam[axis_q(k, -1)];
+        a1[k] = p - m;
+        a2[k] = p + m;
+    }
+    // diagonals of the planes (x,y), (x,z), (y,z): total, c_k c_l sum, and
+    // the first moments along k and l
+    C T[3], Xc[3], Dk[3], Dl[3];
+#pragma unroll
+    for (int pl = 0; pl < 3; pl++) {
+        const int k = pl == 2 ? 1 : 0, l = pl == 0 ? 1 : 2;
+        const C pp = lam[diag_q(k, l, 1, 1)], pm = lam[diag_q(k, l, 1, -1)];
+        const C mp = lam[diag_q(k, l, -1, 1)], mm = lam[diag_q(k, l, -1, -1)];
+        const C s1 = pp + mm, s2 = pm + mp, t1 = pp - mm, t2 = pm - mp;
+        T[pl] = s1 + s2;
+        Xc[pl] = s1 - s2;
+        Dk[pl] = t1 + t2;
+        Dl[pl] = t1 - t2;
+    }
+    const C d1[3] = {Dk[0] + Dk[1], Dl[0] + Dk[2], Dl[1] + Dl[2]};
+    const C d2[3] = {T[0] + T[1], T[0] + T[2], T[1] + T[2]};
+    C m1[3], Pd[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        m1[k] = fma_k(a1[k], w1, d1[k] * w2);
+        Pd[k] = fma_k(a2[k], w1, d2[k] * w2);
+    }
+    const C Pxy = Xc[0] * w2, Pxz = Xc[1] * w2, Pyz = Xc[2] * w2;
+    const C m0 = fma_k(lam[rest_q()], w0,
+                               fma_k(a2[0] + a2[1] + a2[2], w1, (T[0] + T[1] + T[2]) * w2));
+    // P u, u.P u, |u|^2
+    const C Pu[3] = {fma_k(Pd[0], u[0], fma_k(Pxy, u[1], Pxz * u[2])),
+                     fma_k(Pxy, u[0], fma_k(Pd[1], u[1], Pyz * u[2])),
+                     fma_k(Pxz, u[0], fma_k(Pyz, u[1], Pd[2] * u[2]))};
+    const C uPu = fma_k(u[0], Pu[0], fma_k(u[1], Pu[1], u[2] * Pu[2]));
+    const C usq = fma_k(u[0], u[0], fma_k(u[1], u[1], u[2] * u[2]));
+    const C A = fma_k((C)-4.5, uPu, m0 * fma_k((C)1.5, usq, (C)1));
+    C oG[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++)
+        oG[k] = omega * fma_k((C)9, Pu[k], (C)3 * fma_k(-m0, u[k], m1[k]));
+    const C oA = omega * A, om1 = (C)1 - omega;
+    // omega (A + G.c_q) for each population: the axis values oA +- oG_k, the
+    // diagonals from them
+    C ax[3][2];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        ax[k][0] = oA + oG[k];
+        ax[k][1] = oA - oG[k];
+    }
 #pragma unroll
     for (int q = 0; q < 19; q++) {
-        const C wl = (C)wq(q) * lam[q];
-        const C cu = dot_c<C>(q, ux, uy, uz);
-        A += wl * ((C)1 - (C)4.5 * cu * cu + (C)1.5 * usq);
-        const C s = wl * ((C)3 + (C)9 * cu);
-        Bx = add_c(Bx, cx(q), s);
-        By = add_c(By, cy(q), s);
-        Bz = add_c(Bz, cz(q), s);
-        Sw += wl;
+        const int n = (cx(q) != 0) + (cy(q) != 0) + (cz(q) != 0);
+        C t;
+        if (n == 0) {
+            t = oA;
+        } else {
+            const int k = cx(q) != 0 ? 0 : (cy(q) != 0 ? 1 : 2);
+            t = ax[k][comp_c(q, k) > 0 ? 0 : 1];
+            if (n == 2) {
+                const int l = (k == 0 && cy(q) != 0) ? 1 : 2;
+                t = comp_c(q, l) > 0 ? t + oG[l] : t - oG[l];
+            }
+        }
+        mu[q] = fma_k(om1, lam[q], t);
     }
-    const C Gx = Bx - (C)3 * Sw * ux, Gy = By - (C)3 * Sw * uy, Gz = Bz - (C)3 * Sw * uz;
-    const C om1 = (C)1 - omega;
-#pragma unroll
-    for (int q = 0; q < 19; q++)
-        mu[q] = om1 * lam[q] + omega * add_c(add_c(add_c(A, cx(q), Gx), cy(q), Gy), cz(q), Gz);
 }
 
 // one adjoint step: lambda^{t+1}_i(x) = M^{t+1}_i(x + c_i) (shifted loads),
@@ -361,10 +433,12 @@ hipError_t launch_tape3(bgk_ctx *c, C *tape0, C *tape1, C *tape2, hipStream_t s)
 
 // Three-step passes of the adjoint's forward and backward: where
 // three_step_ok holds, lambda (in the compute kind) takes 32-bit offsets and
-// a pass has a full round of workgroups (one per CU; at 128^3, 208
-// workgroups, the adjoint was 2 % slower with them, at 256^3 the backward
-// 10 % faster).  Of n steps, n3 such passes (a remainder of one goes to two
-// two-step passes, as in bgk_step).
+// a pass has at least half a round of workgroups (one per CU), as the plain
+// forward's passes.  At 128^3 (208 workgroups) the backward with them takes
+// 38.3 ms per 500 steps against 39.3 with two-step passes (with them in the
+// sweep or in the re-tape alone, 39.3); at 256^3 the backward is 10 % faster.
+// Of n steps, n3 such passes (a remainder of one goes to two two-step
+// passes, as in bgk_step).
 template <typename C>
 int adjoint_triples(const Lattice &L, int n) {
     if (!three_step_ok(L) || !offsets_fit_32(L, sizeof(C))) return 0;
@@ -372,7 +446,7 @@ int adjoint_triples(const Lattice &L, int n) {
     const int zc = three_step_zc(L, T::X, T::Y, T::ZC);
     const long long nb = (long long)((L.nx + T::X - 1) / T::X) * ((L.ny + T::Y - 1) / T::Y) *
                          ((L.nz + zc - 1) / zc);
-    if (L.two_step != 2 && nb < L.ncu) return 0;
+    if (L.two_step != 2 && 2 * nb < L.ncu) return 0;
     int n3 = n / 3;
     if (n % 3 == 1 && n3 > 0 && two_step_ok(L)) n3--;
     return n3;

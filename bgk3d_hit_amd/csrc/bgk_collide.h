@@ -1202,7 +1202,11 @@ __device__ __forceinline__ void three_step_pass(const Pops<S> &src, const Pops<S
     auto site_of = [&](int p) {
         return (size_t)gx + (size_t)g.nx * gy + (size_t)g.nxy * p;
     };
-    auto load_plane = [&](int z, S(&raw)[19], X_t &xr) {
+    // the loads of populations [QA, QB) of plane z (and, with the first part,
+    // the per-site operation's inputs of the first stage); the next plane's
+    // go out in parts after the stages' collisions, as in k_collide3 (Load3)
+    auto load_part = [&](int z, S(&raw)[19], X_t &xr, auto qa, auto qb) {
+        constexpr int QA = decltype(qa)::value, QB = decltype(qb)::value;
         const int p = wrapz(z);
         const int zm = (p == 0) ? g.nz - 1 : p - 1;
         const int zp = (p == g.nz - 1) ? 0 : p + 1;
@@ -1211,11 +1215,16 @@ __device__ __forceinline__ void three_step_pass(const Pops<S> &src, const Pops<S
                                              raw_rsrc(b0 + (ptrdiff_t)p * plane),
                                              raw_rsrc(b0 + (ptrdiff_t)zp * plane)};
 #pragma unroll
-        for (int q = 0; q < 19; q++)
+        for (int q = QA; q < QB; q++)
             raw[q] = buf_ld<S>(R[1 - SGN * cz(q)], X[1 - SGN * cx(q)] + Y[1 - SGN * cy(q)],
                                (uint32_t)q * g.pstride);
-        xr = op.extra_first(site_of(p));
+        if constexpr (QA == 0) xr = op.extra_first(site_of(p));
     };
+    using LP = Load3<sizeof(C)>;
+    std::integral_constant<int, 0> q0;
+    std::integral_constant<int, LP::A> qA;
+    std::integral_constant<int, LP::B> qB;
+    std::integral_constant<int, 19> q19;
     // pulled value of population q for the site (sy, sx) of a state whose
     // x/y-moving populations are in `lds` (newest plane group 0, slot pb of
     // the plane two before); the z-only ones from the column's registers
@@ -1241,7 +1250,7 @@ __device__ __forceinline__ void three_step_pass(const Pops<S> &src, const Pops<S
             asm volatile("" : "+v"(f[q]));  // see two_step_pass
         }
         const X_t x1 = xcur;
-        if (it + 1 < nit) load_plane(z + 1, nxt, xnxt);
+        const bool more = it + 1 < nit;
         const bool do2 = mid && it >= 2, do3 = owner && it >= 4;
         X_t x2{}, x3{};
         if (do2) x2 = op.extra_second(site_of(wrapz(z - 1)));
@@ -1255,6 +1264,7 @@ __device__ __forceinline__ void three_step_pass(const Pops<S> &src, const Pops<S
             for (int q = 0; q < 19; q++)
                 if (pgroup<SGN>(q) == 0) l1[pbase(0) + pslot<SGN>(q)][ey][ex] = s1[q];
         }
+        if (more) load_part(z + 1, nxt, xnxt, q0, qA);
         lds_barrier();
         if (do2) {  // S2 at plane z-1
             C f2[19], b[19];
@@ -1268,6 +1278,7 @@ __device__ __forceinline__ void three_step_pass(const Pops<S> &src, const Pops<S
             for (int q = 0; q < 19; q++)
                 if (pgroup<SGN>(q) == 0) l2[pbase(0) + pslot<SGN>(q)][ey - 1][ex - 1] = s2[q];
         }
+        if (more) load_part(z + 1, nxt, xnxt, qA, qB);
         lds_barrier();
         if (active) {
 #pragma unroll
@@ -1289,6 +1300,7 @@ __device__ __forceinline__ void three_step_pass(const Pops<S> &src, const Pops<S
                 buf_st<S>(Storage<S>::template store<C>(b[q]), R, X[1] + Y[1],
                           (uint32_t)q * g.pstride);
         }
+        if (more) load_part(z + 1, nxt, xnxt, qB, q19);
         lds_barrier();
         if (do2) {
 #pragma unroll
@@ -1307,7 +1319,7 @@ __device__ __forceinline__ void three_step_pass(const Pops<S> &src, const Pops<S
 
     S ra[19];
     X_t xa;
-    load_plane(za - 2, ra, xa);
+    load_part(za - 2, ra, xa, q0, q19);
     if constexpr (Op::PINGPONG) {
         S rb[19];
         X_t xb;
