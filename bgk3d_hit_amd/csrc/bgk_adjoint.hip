@@ -223,6 +223,7 @@ template <typename C>
 struct AdjointOp {
     static constexpr int SGN = -1;
     static constexpr bool PINGPONG = true;
+    static constexpr bool KEEP_STATE = true;
     struct Extra {
         C u[3];
     };
@@ -260,6 +261,7 @@ template <typename C>
 struct AdjointOp3 {
     static constexpr int SGN = -1;
     static constexpr bool PINGPONG = true;
+    static constexpr bool KEEP_STATE = true;
     struct Extra {
         C u[3];
     };
@@ -293,11 +295,12 @@ __global__ __launch_bounds__(collide3_threads(TX, TY), 1) void k_adjoint3(Pops<C
     three_step_pass<C, C, TX, TY, AdjointOp3<C>>(src, dst, g, op);
 }
 
-// three taped forward steps in one pass (u of each step to its tape slice)
-template <typename S, typename C, int TX, int TY>
+// three taped forward steps in one pass (u of each step to its tape slice);
+// KEEP = false: the final state is not stored (see CollideOp)
+template <typename S, typename C, int TX, int TY, bool KEEP = true>
 __global__ __launch_bounds__(collide3_threads(TX, TY), 1) void k_tape3(
-    Pops<S> src, Pops<S> dst, Geo2 g, CollideOp3<C, true, true> op) {
-    three_step_pass<S, C, TX, TY, CollideOp3<C, true, true>>(src, dst, g, op);
+    Pops<S> src, Pops<S> dst, Geo2 g, CollideOp3<C, true, true, KEEP> op) {
+    three_step_pass<S, C, TX, TY, CollideOp3<C, true, true, KEEP>>(src, dst, g, op);
 }
 
 // dJ/du0 = sum_i lambda^0_i w_i (3 c_i + 9 (c_i.u0) c_i - 3 u0),
@@ -401,34 +404,62 @@ hipError_t launch_tape_step(bgk_ctx *c, C *tape, hipStream_t s) {
     return hipGetLastError();
 }
 
+// The lattice as a pass of the backward sizes its z chunks (two_step_zc,
+// three_step_zc minimise rounds of one workgroup per CU times the plane
+// iterations of a chunk): the re-tape and the adjoint sweep run on two
+// streams at once, so each of their passes has about half of the CUs.  At
+// 128^3 that takes 32-plane chunks (104 workgroups, 36 plane iterations)
+// instead of 16-plane ones (208 workgroups, 20 iterations each, two rounds
+// when shared).  ncu = 0: the device's count.
+inline Lattice with_cus(const Lattice &L, int ncu) {
+    Lattice s = L;
+    if (ncu > 0) s.ncu = ncu;
+    return s;
+}
+
 // two taped forward steps (slices tape0, tape1) in one pass
-template <typename S, typename C, bool O32>
-hipError_t launch_tape2_o(bgk_ctx *c, C *tape0, C *tape1, hipStream_t s) {
+template <typename S, typename C, bool O32, bool KEEP>
+hipError_t launch_tape2_k(bgk_ctx *c, C *tape0, C *tape1, hipStream_t s, int ncu) {
     const Lattice &L = c->lat;
     using T = Tile2<S, C, O32>;
     int nblocks;
-    const Geo2 g = make_geo2(L, T::X, T::Y, two_step_zc(L, T::X, T::Y, T::ZC), &nblocks);
-    const CollideOp<C, true, true> op{(C)c->omega, (C)1, tape0, tape1, (size_t)L.nsite};
-    hipLaunchKernelGGL((k_collide2<S, C, O32, T::X, T::Y, T::MINB, true, false, true>), dim3(nblocks),
-                       dim3(collide2_threads(T::X, T::Y)), 0, s, pops_of<S>(L, c->cur),
-                       pops_of<S>(L, 1 - c->cur), g, op);
+    const Geo2 g =
+        make_geo2(L, T::X, T::Y, two_step_zc(with_cus(L, ncu), T::X, T::Y, T::ZC), &nblocks);
+    const CollideOp<C, true, true, KEEP> op{(C)c->omega, (C)1, tape0, tape1, (size_t)L.nsite};
+    hipLaunchKernelGGL((k_collide2<S, C, O32, T::X, T::Y, T::MINB, true, false, true, KEEP>),
+                       dim3(nblocks), dim3(collide2_threads(T::X, T::Y)), 0, s,
+                       pops_of<S>(L, c->cur), pops_of<S>(L, 1 - c->cur), g, op);
     c->cur ^= 1;
     return hipGetLastError();
 }
+// keep = false: the final state is not stored (the last pass of a re-tape)
+template <typename S, typename C, bool O32>
+hipError_t launch_tape2_o(bgk_ctx *c, C *tape0, C *tape1, hipStream_t s, int ncu, bool keep) {
+    return keep ? launch_tape2_k<S, C, O32, true>(c, tape0, tape1, s, ncu)
+                : launch_tape2_k<S, C, O32, false>(c, tape0, tape1, s, ncu);
+}
 
 // three taped forward steps (slices tape0..tape2) in one pass
-template <typename S, typename C>
-hipError_t launch_tape3(bgk_ctx *c, C *tape0, C *tape1, C *tape2, hipStream_t s) {
+template <typename S, typename C, bool KEEP>
+hipError_t launch_tape3_k(bgk_ctx *c, C *tape0, C *tape1, C *tape2, hipStream_t s, int ncu) {
     const Lattice &L = c->lat;
     using T = Tile3<S, C>;
     int nblocks;
-    const Geo2 g = make_geo2(L, T::X, T::Y, three_step_zc(L, T::X, T::Y, T::ZC), &nblocks);
-    const CollideOp3<C, true, true> op{(C)c->omega, (C)1, tape0, tape1, tape2, (size_t)L.nsite};
-    hipLaunchKernelGGL((k_tape3<S, C, T::X, T::Y>), dim3(nblocks),
+    const Geo2 g =
+        make_geo2(L, T::X, T::Y, three_step_zc(with_cus(L, ncu), T::X, T::Y, T::ZC), &nblocks);
+    const CollideOp3<C, true, true, KEEP> op{(C)c->omega, (C)1, tape0, tape1, tape2,
+                                             (size_t)L.nsite};
+    hipLaunchKernelGGL((k_tape3<S, C, T::X, T::Y, KEEP>), dim3(nblocks),
                        dim3(collide3_threads(T::X, T::Y)), 0, s, pops_of<S>(L, c->cur),
                        pops_of<S>(L, 1 - c->cur), g, op);
     c->cur ^= 1;
     return hipGetLastError();
+}
+template <typename S, typename C>
+hipError_t launch_tape3(bgk_ctx *c, C *tape0, C *tape1, C *tape2, hipStream_t s, int ncu,
+                        bool keep) {
+    return keep ? launch_tape3_k<S, C, true>(c, tape0, tape1, tape2, s, ncu)
+                : launch_tape3_k<S, C, false>(c, tape0, tape1, tape2, s, ncu);
 }
 
 // Three-step passes of the adjoint's forward and backward: where
@@ -440,40 +471,47 @@ hipError_t launch_tape3(bgk_ctx *c, C *tape0, C *tape1, C *tape2, hipStream_t s)
 // Of n steps, n3 such passes (a remainder of one goes to two two-step
 // passes, as in bgk_step).
 template <typename C>
-int adjoint_triples(const Lattice &L, int n) {
+int adjoint_triples(const Lattice &L, int n, int ncu) {
     if (!three_step_ok(L) || !offsets_fit_32(L, sizeof(C))) return 0;
     using T = Tile3<C, C>;
-    const int zc = three_step_zc(L, T::X, T::Y, T::ZC);
+    const Lattice Lz = with_cus(L, ncu);
+    const int zc = three_step_zc(Lz, T::X, T::Y, T::ZC);
     const long long nb = (long long)((L.nx + T::X - 1) / T::X) * ((L.ny + T::Y - 1) / T::Y) *
                          ((L.nz + zc - 1) / zc);
-    if (L.two_step != 2 && 2 * nb < L.ncu) return 0;
+    if (L.two_step != 2 && 2 * nb < Lz.ncu) return 0;
     int n3 = n / 3;
     if (n % 3 == 1 && n3 > 0 && two_step_ok(L)) n3--;
     return n3;
 }
 
 // taped forward steps on stream s: slice k of `tape` records step k of the
-// n steps
+// n steps (z chunks for ncu CUs, with_cus).  keep_last = false: the last
+// fused pass does not store its state (a re-tape's end, whose state nothing
+// reads; single steps always store).
 template <typename S, typename C>
-int run_taped(bgk_ctx *c, C *tape, int n, hipStream_t s) {
+int run_taped(bgk_ctx *c, C *tape, int n, hipStream_t s, int ncu, bool keep_last = true) {
     const Lattice &L = c->lat;
     const size_t slice = (size_t)3 * L.nsite;
     hipEvent_t ev;
+    const int n3 = adjoint_triples<C>(L, n, ncu);
+    const bool two = two_step_ok(L);
     int k = 0;
-    for (int n3 = adjoint_triples<C>(L, n); n3 > 0; n3--, k += 3) {
+    for (int i = 0; i < n3; i++, k += 3) {
+        const bool keep = keep_last || k + 3 < n;
         BGK_HIP(timing_begin(c, TCLS_TAPE, s, &ev));
         BGK_HIP((launch_tape3<S, C>(c, tape + k * slice, tape + (k + 1) * slice,
-                                    tape + (k + 2) * slice, s)));
+                                    tape + (k + 2) * slice, s, ncu, keep)));
         BGK_HIP(timing_end(c, TCLS_TAPE, s, ev));
     }
-    if (two_step_ok(L)) {
+    if (two) {
         const bool o32 = offsets_fit_32(L, sizeof(S));
         for (; k + 1 < n; k += 2) {
+            const bool keep = keep_last || k + 2 < n;
             BGK_HIP(timing_begin(c, TCLS_TAPE, s, &ev));
             BGK_HIP(o32 ? (launch_tape2_o<S, C, true>(c, tape + k * slice,
-                                                      tape + (k + 1) * slice, s))
+                                                      tape + (k + 1) * slice, s, ncu, keep))
                         : (launch_tape2_o<S, C, false>(c, tape + k * slice,
-                                                       tape + (k + 1) * slice, s)));
+                                                       tape + (k + 1) * slice, s, ncu, keep)));
             BGK_HIP(timing_end(c, TCLS_TAPE, s, ev));
         }
     }
@@ -487,11 +525,11 @@ int run_taped(bgk_ctx *c, C *tape, int n, hipStream_t s) {
 
 template <typename C, bool O32>
 hipError_t launch_adjoint2_o(bgk_ctx *c, const C *lin, C *lout, const C *tape0,
-                             const C *tape1) {
+                             const C *tape1, int ncu) {
     const Lattice &L = c->lat;
     using T = Tile2<C, C, O32>;
     int nblocks;
-    Geo2 g = make_geo2(L, T::X, T::Y, two_step_zc(L, T::X, T::Y, T::ZC), &nblocks);
+    Geo2 g = make_geo2(L, T::X, T::Y, two_step_zc(with_cus(L, ncu), T::X, T::Y, T::ZC), &nblocks);
     // lambda: the lattice's interleaved layout in the compute kind
     g.pstride = (unsigned)(L.nx * sizeof(C));
     Pops<C> a, b;
@@ -507,11 +545,12 @@ hipError_t launch_adjoint2_o(bgk_ctx *c, const C *lin, C *lout, const C *tape0,
 
 template <typename C>
 hipError_t launch_adjoint3(bgk_ctx *c, const C *lin, C *lout, const C *tape0, const C *tape1,
-                           const C *tape2) {
+                           const C *tape2, int ncu) {
     const Lattice &L = c->lat;
     using T = Tile3<C, C>;
     int nblocks;
-    Geo2 g = make_geo2(L, T::X, T::Y, three_step_zc(L, T::X, T::Y, T::ZC), &nblocks);
+    Geo2 g =
+        make_geo2(L, T::X, T::Y, three_step_zc(with_cus(L, ncu), T::X, T::Y, T::ZC), &nblocks);
     // lambda: the lattice's interleaved layout in the compute kind
     g.pstride = (unsigned)(L.nx * sizeof(C));
     Pops<C> a, b;
@@ -529,17 +568,18 @@ template <typename C>
 hipError_t launch_adjoint_step(bgk_ctx *c, const C *lin, C *lout, const C *tape);
 
 // adjoint steps n-1 .. 0 of a segment whose tape slice k is step k; lin
-// holds M after step n and ends holding M after step 0
+// holds M after step n and ends holding M after step 0 (z chunks for ncu
+// CUs, with_cus)
 template <typename C>
-int adjoint_sweep(bgk_ctx *c, C *&lin, C *&lout, const C *tape, int n) {
+int adjoint_sweep(bgk_ctx *c, C *&lin, C *&lout, const C *tape, int n, int ncu) {
     const Lattice &L = c->lat;
     const size_t slice = (size_t)3 * L.nsite;
     hipEvent_t ev;
     int k = n - 1;
-    for (int n3 = adjoint_triples<C>(L, n); n3 > 0; n3--, k -= 3) {
+    for (int n3 = adjoint_triples<C>(L, n, ncu); n3 > 0; n3--, k -= 3) {
         BGK_HIP(timing_begin(c, TCLS_ADJOINT, c->s0, &ev));
         BGK_HIP(launch_adjoint3<C>(c, lin, lout, tape + k * slice, tape + (k - 1) * slice,
-                                   tape + (k - 2) * slice));
+                                   tape + (k - 2) * slice, ncu));
         BGK_HIP(timing_end(c, TCLS_ADJOINT, c->s0, ev));
         std::swap(lin, lout);
     }
@@ -548,9 +588,9 @@ int adjoint_sweep(bgk_ctx *c, C *&lin, C *&lout, const C *tape, int n) {
         for (; k >= 1; k -= 2) {
             BGK_HIP(timing_begin(c, TCLS_ADJOINT, c->s0, &ev));
             BGK_HIP(o32 ? (launch_adjoint2_o<C, true>(c, lin, lout, tape + k * slice,
-                                                      tape + (k - 1) * slice))
+                                                      tape + (k - 1) * slice, ncu))
                         : (launch_adjoint2_o<C, false>(c, lin, lout, tape + k * slice,
-                                                       tape + (k - 1) * slice)));
+                                                       tape + (k - 1) * slice, ncu)));
             BGK_HIP(timing_end(c, TCLS_ADJOINT, c->s0, ev));
             std::swap(lin, lout);
         }
@@ -587,9 +627,9 @@ int first_pass_steps(const Lattice &L, int n) {
 }
 
 template <typename C>
-int taped_first_pass(const Lattice &L, int n) {
+int taped_first_pass(const Lattice &L, int n, int ncu) {
     if (n <= 0) return 0;
-    if (adjoint_triples<C>(L, n) > 0) return 3;
+    if (adjoint_triples<C>(L, n, ncu) > 0) return 3;
     return (two_step_ok(L) && n >= 2) ? 2 : 1;
 }
 
@@ -666,13 +706,14 @@ int adjoint_t(bgk_ctx *c, const double *u0, int T, int K, int levels, double *J,
                 : t >= last_s0 ? tape[0] + (size_t)(t - last_s0) * 3 * N
                                : tape[1] + (size_t)(t - taped_s0) * 3 * N;
         auto run = [&](int m) {
-            return taped ? run_taped<S, C>(c, tp, m, c->s0) : step_periodic(c, m, false, c->s0);
+            return taped ? run_taped<S, C>(c, tp, m, c->s0, L.ncu)
+                         : step_periodic(c, m, false, c->s0);
         };
         int done = 0;
         if (t % K2 == 0) {
             const int k = c->cur;
             ck[t / K2] = c->lat.pop[k][0];
-            done = taped ? taped_first_pass<C>(L, n) : first_pass_steps(L, n);
+            done = taped ? taped_first_pass<C>(L, n, L.ncu) : first_pass_steps(L, n);
             if ((r = run(done))) return r;
             a.set_copy(k, spare.back());
             spare.pop_back();
@@ -708,6 +749,9 @@ int adjoint_t(bgk_ctx *c, const double *u0, int T, int K, int levels, double *J,
         for (int j = nf - 1; j >= 0; j--)
             segs.push_back(Seg{cs, j, c0 + j * K, std::min(c1, c0 + j * K + K)});
     }
+    // CUs a re-tape or a sweep beside one has (with_cus): half of them
+    // when re-tapes run (more than two segments)
+    const int ncu_shared = segs.size() > 2 ? std::max(1, L.ncu / 2) : L.ncu;
     int prepared_cs = ncoarse;  // coarse segment whose fine checkpoints are in `fine`
     auto retape = [&](int k) -> int {
         const Seg &g = segs[k];
@@ -726,13 +770,15 @@ int adjoint_t(bgk_ctx *c, const double *u0, int T, int K, int levels, double *J,
             prepared_cs = g.cs;
         }
         // the first taped pass reads the checkpoint in place
-        const int n = g.s1 - g.s0, kk = c->cur, first = taped_first_pass<C>(L, n);
+        const int n = g.s1 - g.s0, kk = c->cur, first = taped_first_pass<C>(L, n, ncu_shared);
         void *mine = c->lat.pop[kk][0];
         a.set_copy(kk, M > 1 ? fine[g.j] : ck[g.cs]);
-        if ((rr = run_taped<S, C>(c, tape[k % 2], first, s1))) return rr;
+        // the re-tape's final state is not stored (nothing reads it)
+        if ((rr = run_taped<S, C>(c, tape[k % 2], first, s1, ncu_shared, n > first))) return rr;
         a.set_copy(kk, mine);
         if (n > first &&
-            (rr = run_taped<S, C>(c, tape[k % 2] + (size_t)first * 3 * N, n - first, s1)))
+            (rr = run_taped<S, C>(c, tape[k % 2] + (size_t)first * 3 * N, n - first, s1,
+                                  ncu_shared, false)))
             return rr;
         a.recomputed += g.s1 - g.s0;
         BGK_HIP(hipEventRecord(ev_tape.e[k % 2], s1));
@@ -742,7 +788,9 @@ int adjoint_t(bgk_ctx *c, const double *u0, int T, int K, int levels, double *J,
     const int nsegs = (int)segs.size();
     for (int k = 0; k < nsegs; k++) {
         if (k >= 2) BGK_HIP(hipStreamWaitEvent(c->s0, ev_tape.e[k % 2], 0));
-        if ((r = adjoint_sweep<C>(c, lin, lout, tape[k % 2], segs[k].s1 - segs[k].s0)))
+        // sweeps 1 .. nsegs-2 run beside a re-tape, the first and the last alone
+        const int ncu_k = (k >= 1 && k + 1 < nsegs) ? ncu_shared : L.ncu;
+        if ((r = adjoint_sweep<C>(c, lin, lout, tape[k % 2], segs[k].s1 - segs[k].s0, ncu_k)))
             return r;
         BGK_HIP(hipEventRecord(ev_sweep.e[k % 2], c->s0));
         if (k + 2 < nsegs && (r = retape(k + 2))) return r;

@@ -497,10 +497,13 @@ __device__ __forceinline__ void write_faces(void *faces, const Geo2 &g, const S 
 // loaded with a plane's populations (prefetched one plane ahead).
 
 // stream-collide; TAPE: record u = j/rho of both steps (slices tape0, tape1);
-// SH: shifted storage only (collide_site<C, true>, cte1 = 1)
-template <typename C, bool TAPE, bool SH = false>
+// SH: shifted storage only (collide_site<C, true>, cte1 = 1); KEEP = false:
+// the pass does not store its final state (a taped re-run's last pass, whose
+// state nothing reads; only its tape is kept)
+template <typename C, bool TAPE, bool SH = false, bool KEEP = true>
 struct CollideOp {
     static constexpr int SGN = 1;
+    static constexpr bool KEEP_STATE = KEEP;
     // two load register sets (see two_step_pass); the taping variant keeps
     // one, its tape pointers would not fit beside the second set
     static constexpr bool PINGPONG = !TAPE;
@@ -707,7 +710,9 @@ __device__ __forceinline__ void two_step_pass(const Pops<S> &src, const Pops<S> 
             C b[19];
             op.second(f2, b, x2, site_of(p2));
             const ptrdiff_t zo = (ptrdiff_t)(p2 + (ZM == 1 ? 0 : g.zoff)) * plane;
-            if constexpr (BUF) {
+            if constexpr (!Op::KEEP_STATE) {
+                // no store of the final state
+            } else if constexpr (BUF) {
                 const __amdgpu_buffer_rsrc_t R = raw_rsrc(dst.p[0] + zo);
 #pragma unroll
                 for (int q = 0; q < 19; q++)
@@ -756,11 +761,12 @@ __device__ __forceinline__ void two_step_pass(const Pops<S> &src, const Pops<S> 
 }
 
 template <typename S, typename C, bool O32, int TX, int TY, int MINB, bool TAPE,
-          bool FACES = false, bool SH = false>
+          bool FACES = false, bool SH = false, bool KEEP = true>
 __global__ __launch_bounds__(collide2_threads(TX, TY), MINB) void k_collide2(
-    Pops<S> src, Pops<S> dst, Geo2 g, CollideOp<C, TAPE, SH> op) {
+    Pops<S> src, Pops<S> dst, Geo2 g, CollideOp<C, TAPE, SH, KEEP> op) {
     __shared__ S lds[20][TY + 2][TX + 2];
-    two_step_pass<S, C, O32, TX, TY, CollideOp<C, TAPE, SH>, FACES, O32>(src, dst, g, op, lds);
+    two_step_pass<S, C, O32, TX, TY, CollideOp<C, TAPE, SH, KEEP>, FACES, O32>(src, dst, g, op,
+                                                                              lds);
 }
 
 // ------------------------------------------------------ three steps per pass
@@ -1107,9 +1113,10 @@ __global__ __launch_bounds__(collide3_threads(TX, TY), 1) void k_collide3(Pops<S
 // first, second and third run at S1, S2 and S3; own = the site is one of the
 // workgroup's own sites of its chunk.  TAPE: record u = j/rho of the three
 // steps at the own sites (slices tape0..tape2).
-template <typename C, bool TAPE, bool SH>
+template <typename C, bool TAPE, bool SH, bool KEEP = true>
 struct CollideOp3 {
     static constexpr int SGN = 1;
+    static constexpr bool KEEP_STATE = KEEP;  // as CollideOp
     static constexpr bool PINGPONG = !TAPE;
     struct Extra {};
     C omega, cte1;
@@ -1294,11 +1301,14 @@ __device__ __forceinline__ void three_step_pass(const Pops<S> &src, const Pops<S
                 f3[q] = Storage<S>::template load<C>(
                     pull(l2, q, ey - 1, ex - 1, pb, s2[QN], r2O_2, r2R_1));
             op.third(f3, b, x3, site_of(wrapz(z - 2)));
-            const __amdgpu_buffer_rsrc_t R = raw_rsrc(dst.p[0] + (ptrdiff_t)wrapz(z - 2) * plane);
+            if constexpr (Op::KEEP_STATE) {
+                const __amdgpu_buffer_rsrc_t R =
+                    raw_rsrc(dst.p[0] + (ptrdiff_t)wrapz(z - 2) * plane);
 #pragma unroll
-            for (int q = 0; q < 19; q++)
-                buf_st<S>(Storage<S>::template store<C>(b[q]), R, X[1] + Y[1],
-                          (uint32_t)q * g.pstride);
+                for (int q = 0; q < 19; q++)
+                    buf_st<S>(Storage<S>::template store<C>(b[q]), R, X[1] + Y[1],
+                              (uint32_t)q * g.pstride);
+            }
         }
         if (more) load_part(z + 1, nxt, xnxt, qB, q19);
         lds_barrier();
