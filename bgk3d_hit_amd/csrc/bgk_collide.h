@@ -809,13 +809,25 @@ constexpr int collide3_threads(int tx, int ty) { return tx * (ty + 4) + 64; }
 // ms per pass against all 19 loads at the top of the iteration: FP32 5.09-5.33
 // -> 4.56-4.68 (split 10/9; 12/7 4.64, 8/11 4.61-4.63, 10/5/4 4.62-4.66,
 // 7/6/6 4.75-4.88), FP16 4.43 -> 4.12-4.24, BF16 4.48-4.53 -> 4.03-4.15,
-// FP64 10.94 -> 9.80-9.82 (7/6/6; 10/9 10.18-10.51).
-template <int CSIZE>
+// FP64 10.94 -> 9.80-9.82 (7/6/6; 10/9 10.18-10.51).  RED (the steps +
+// diagnostics pass, whose second stage also stores S2 and sums its moments
+// and whose third stage stores nothing): [10, 19) after the third stage,
+// 5.25-5.36 -> 5.01-5.08 ms per pass in FP32 (all 19 after the first
+// stage: 5.38-5.50).
+template <int CSIZE, bool RED = false>
 struct Load3 {
     static constexpr int A = 10, B = 19;
 };
+template <int CSIZE>
+struct Load3<CSIZE, true> {
+    static constexpr int A = 10, B = 10;
+};
+template <bool RED>
+struct Load3<8, RED> {
+    static constexpr int A = 7, B = 13;
+};
 template <>
-struct Load3<8> {
+struct Load3<8, true> {
     static constexpr int A = 7, B = 13;
 };
 
@@ -936,7 +948,7 @@ __global__ __launch_bounds__(collide3_threads(TX, TY), 1) void k_collide3(Pops<S
     // loads of a workgroup queue behind each other and the youngest waves
     // of each SIMD wait thousands of cycles to issue theirs before they can
     // start the first stage.
-    using LP = Load3<sizeof(C)>;
+    using LP = Load3<sizeof(C), RED>;
     std::integral_constant<int, 0> q0;
     std::integral_constant<int, LP::A> qA;
     std::integral_constant<int, LP::B> qB;
