@@ -184,11 +184,17 @@ static int record_faces(bgk_ctx *c, hipStream_t s) {
 // (copy 1-a) records S_T into slot[1-a] and writes the faces of its
 // intermediate S_{T+1} into slot[a] from inside the pass; a three-step pass
 // to S_{T+3} writes the faces of S_{T+1} into slot[1-a] and those of S_{T+2}
-// into slot[a] from inside the pass.
+// into slot[a] from inside the pass.  The images are read only through
+// slot[cur] (the dissipation) and a later single step keeps slot[1-cur]; so
+// after a call of n steps only S_{n-2} and S_{n-1} must be recorded, and a
+// pass writes images only if it ends at step n-1 or later (the last pass of
+// the call, and the one before it when a single step ends the call).  The
+// passes before go without them (512^3 three-step pass with images: FP32
+// 5.04 against 4.48 ms, FP16 4.85 against 3.88 ms).
 int step_periodic(bgk_ctx *c, int nsteps, bool count, hipStream_t stream) {
     hipStream_t s = stream ? stream : c->s0;
     std::vector<Box> all{full_box(c->lat)};
-    const bool images = (c->flags & BGK_FLAG_REFCOMPAT_HALOS) != 0;
+    const bool refimg = (c->flags & BGK_FLAG_REFCOMPAT_HALOS) != 0;
     const int n0 = nsteps;
     if (three_step_ok(c->lat)) {
         // a remainder of one step: two two-step passes rather than a
@@ -197,6 +203,7 @@ int step_periodic(bgk_ctx *c, int nsteps, bool count, hipStream_t stream) {
         if (nsteps % 3 == 1 && n3 > 0 && two_step_ok(c->lat)) n3--;
         for (; n3 > 0; n3--, nsteps -= 3) {
             const int a = c->cur;
+            const bool images = refimg && nsteps - 3 <= 1;
             hipEvent_t ev;
             BGK_HIP(timing_begin(c, TCLS_PASS3, s, &ev));
             BGK_HIP(launch_collide3(c->lat, a, c->omega, images ? c->gcache[1 - a] : nullptr,
@@ -208,6 +215,7 @@ int step_periodic(bgk_ctx *c, int nsteps, bool count, hipStream_t stream) {
     if (two_step_ok(c->lat)) {
         for (; nsteps >= 2; nsteps -= 2) {
             const int a = c->cur;
+            const bool images = refimg && nsteps - 2 <= 1;
             hipEvent_t ev;
             if (images) {
                 BGK_HIP(timing_begin(c, TCLS_HALO, s, &ev));
@@ -222,7 +230,7 @@ int step_periodic(bgk_ctx *c, int nsteps, bool count, hipStream_t stream) {
         }
     }
     for (int t = 0; t < nsteps; t++) {
-        if (images) {
+        if (refimg && nsteps - t <= 2) {
             int r = record_faces(c, s);
             if (r) return r;
         }

@@ -250,7 +250,9 @@ subroutine alloca
   use bgk_host
   implicit none
   integer(c_int) :: flags
-  flags = 0
+  ! kernel events per launch: the task log reports the collide launches by
+  ! the time steps each advanced
+  flags = BGK_FLAG_TIMING
   if (diag_c == BGK_DIAG_REFCOMPAT) flags = ior(flags, BGK_FLAG_REFCOMPAT_HALOS)
   call bgk_check(bgk_create(int(l, c_int), int(m, c_int), int(n, c_int), kind_c, &
        BGK_LAYOUT_PERIODIC, flags, real(omega, c_double), ctx), 'bgk_create')
@@ -625,7 +627,7 @@ subroutine finalize(itstart)
   implicit none
   integer, intent(in) :: itstart
   real(dp) :: glups, glups1, knorm, r(2)
-  real(c_double) :: t(8)
+  real(c_double) :: t(8), tp(6), tf(2)
   call date_line(.true.)
   ! per-rank section timers and ratios (SRC/finalize.F90:45-63); one rank
   ! here, so task.000000.log
@@ -653,6 +655,12 @@ subroutine finalize(itstart)
        ratio(real(m + 2, dp) * real(l + 2, dp) * real(itfin, dp) * knorm, time_mp)
   write(38, *) "# Y-MPI time, BW (MB/s) -->", 0.0_dp, 0.0_dp
   write(38, *) "# X-MPI time, BW (MB/s) -->", 0.0_dp, 0.0_dp
+  ! collide launches (and their kernel ms) by the steps one launch advances:
+  ! 1, 2, 3, and 2 with the diagnostics of the result fused in
+  call bgk_check(bgk_timing_passes(ctx, tp, 0_c_int), 'bgk_timing_passes')
+  call bgk_check(bgk_timing_fused(ctx, tf, 0_c_int), 'bgk_timing_fused')
+  write(38, 1300) nint(tp(2)), nint(tp(4)), nint(tp(6)), nint(tf(2))
+  write(38, 1301) tp(1), tp(3), tp(5), tf(1)
   close(38)
   ! SRC/finalize.F90:75-90 (GLUPs of bgk.perf on the second clock, of
   ! stdout on the first, itfin - itstart steps in the numerator)
@@ -693,6 +701,8 @@ subroutine finalize(itstart)
 1216 format(" # Check      ", 2(f7.3, 1x))
 2000 format(" # loop   time", 2(e14.6, 1x))
 3000 format(" # GLUPs      ", 1(e14.6))
+1300 format(" # Passes 1/2/3/2+diag steps", 4(i10, 1x))
+1301 format(" # Pass ms 1/2/3/2+diag steps", 4(e14.6, 1x))
 contains
   real(dp) function ratio(a, b)
     real(dp), intent(in) :: a, b

@@ -187,6 +187,20 @@ module bgk_capi
        integer(c_int), value :: reset
      end function bgk_timing
 
+     integer(c_int) function bgk_timing_passes(ctx, t, reset) bind(c, name='bgk_timing_passes')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), intent(out) :: t(6)
+       integer(c_int), value :: reset
+     end function bgk_timing_passes
+
+     integer(c_int) function bgk_timing_fused(ctx, t, reset) bind(c, name='bgk_timing_fused')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), intent(out) :: t(2)
+       integer(c_int), value :: reset
+     end function bgk_timing_fused
+
      integer(c_int) function bgk_adjoint_ke(ctx, u0, nsteps, ckpt_every, levels, J, grad, &
           stats) bind(c, name='bgk_adjoint_ke')
        import :: c_int, c_ptr, c_double

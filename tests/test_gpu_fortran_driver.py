@@ -155,6 +155,8 @@ def test_driver_two_step_pairs_match_single_steps(gpu, tmp_path):
         assert np.array_equal(dg[row, 1:], [d[1], d[2], d[3], d[0], d[4]]), (it, dg[row], d)
         assert ds[row, 2] == s[1] and ds[row, 3] == s[0], (it, ds[row], s)
     assert dg.shape[0] == itfin // icheck
+    n1, n2, n3, n2d = task_passes(tmp_path)
+    assert n1 + 2 * n2 + 3 * n3 + 2 * n2d == itfin, (n1, n2, n3, n2d)
 
 
 @pytest.mark.gpu
@@ -162,7 +164,8 @@ def test_driver_refcompat_fused_passes(gpu, tmp_path):
     """The reference-compatible diagnostics (stale halo images) with the
     driver's fused passes (7 steps between outputs: a three-step pass and two
     two-step passes) on a 256^3 lattice: equal to the reference call order
-    (boundaries + col_MC every step) exactly."""
+    (boundaries + col_MC every step) exactly.  The task log's launch counts
+    by pass depth show three-step passes and add up to itfin steps."""
     n, omega, itfin, icheck = 256, 1.95, 21, 7
     svisc = (2.0 / omega - 1.0) / 6.0
     deck = (f"&parameters\nlx={n}\nly={n}\nlz={n}\nomega={omega}\nitfin={itfin}\n"
@@ -187,6 +190,18 @@ def test_driver_refcompat_fused_passes(gpu, tmp_path):
             assert ds[row, 2] == s[1] and ds[row, 3] == s[0], (it, ds[row], s)
             row += 1
     assert row == itfin // icheck
+    n1, n2, n3, n2d = task_passes(tmp_path)
+    assert n3 >= itfin // icheck, (n1, n2, n3, n2d)
+    assert n1 + 2 * n2 + 3 * n3 + 2 * n2d == itfin, (n1, n2, n3, n2d)
+
+
+def task_passes(tmp_path):
+    """Collide launches by the steps each advanced (1, 2, 3, 2 + fused
+    diagnostics) from the driver's task.000000.log."""
+    task = (tmp_path / "task.000000.log").read_text().splitlines()
+    rows = [x for x in task if x.startswith(" # Passes 1/2/3/2+diag steps")]
+    assert len(rows) == 1, task
+    return [int(v) for v in rows[0][len(" # Passes 1/2/3/2+diag steps"):].split()]
 
 
 def fortran_e(x, d=6, w=14):
@@ -273,7 +288,8 @@ def test_driver_reference_format_outputs(gpu, tmp_path):
     assert -1.0 <= float(c[1]) <= 1.0
     task = (tmp_path / "task.000000.log").read_text()
     for key in ("# Time for section", "# Ratio", "# Ratio Coll", "# Check",
-                "# Z-MPI time, BW (MB/s)", "# my up task is"):
+                "# Z-MPI time, BW (MB/s)", "# my up task is", "# Passes 1/2/3/2+diag steps",
+                "# Pass ms 1/2/3/2+diag steps"):
         assert key in task, key
     dates = (tmp_path / "time.log").read_text().splitlines()
     assert len(dates) == 2 and all(len(x.split()) == 6 for x in dates), dates
