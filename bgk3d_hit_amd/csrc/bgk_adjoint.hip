@@ -439,11 +439,25 @@ hipError_t launch_tape2_o(bgk_ctx *c, C *tape0, C *tape1, hipStream_t s, int ncu
                 : launch_tape2_k<S, C, O32, false>(c, tape0, tape1, s, ncu);
 }
 
+// Own tiles of the adjoint's three-step passes (taping and adjoint steps):
+// 64x8 in the 4-byte compute kind, where the forward's passes take 64x10.
+// At 128^3 the backward's two streams each size their z chunks for half of
+// the CUs: 64x8 gives 32 tile columns and 4 chunks, 128 workgroups per
+// stream, where 64x10 gave 26 columns and 104 workgroups (48 CUs idle),
+// with the same 36 plane iterations per workgroup and 13 waves each instead
+// of 15: backward 34.3-34.6 -> 29.0-30.8 ms per 500 steps, 256^3 unchanged.
+// FP64 compute keeps 32x10 (32x8 measured slower in the forward).
+template <typename C>
+struct Tile3A {
+    static constexpr bool f64 = sizeof(C) == 8;
+    static constexpr int X = f64 ? 32 : 64, Y = f64 ? 10 : 8, ZC = 64;
+};
+
 // three taped forward steps (slices tape0..tape2) in one pass
 template <typename S, typename C, bool KEEP>
 hipError_t launch_tape3_k(bgk_ctx *c, C *tape0, C *tape1, C *tape2, hipStream_t s, int ncu) {
     const Lattice &L = c->lat;
-    using T = Tile3<S, C>;
+    using T = Tile3A<C>;
     int nblocks;
     const Geo2 g =
         make_geo2(L, T::X, T::Y, three_step_zc(with_cus(L, ncu), T::X, T::Y, T::ZC), &nblocks);
@@ -473,7 +487,7 @@ hipError_t launch_tape3(bgk_ctx *c, C *tape0, C *tape1, C *tape2, hipStream_t s,
 template <typename C>
 int adjoint_triples(const Lattice &L, int n, int ncu) {
     if (!three_step_ok(L) || !offsets_fit_32(L, sizeof(C))) return 0;
-    using T = Tile3<C, C>;
+    using T = Tile3A<C>;
     const Lattice Lz = with_cus(L, ncu);
     const int zc = three_step_zc(Lz, T::X, T::Y, T::ZC);
     const long long nb = (long long)((L.nx + T::X - 1) / T::X) * ((L.ny + T::Y - 1) / T::Y) *
@@ -547,7 +561,7 @@ template <typename C>
 hipError_t launch_adjoint3(bgk_ctx *c, const C *lin, C *lout, const C *tape0, const C *tape1,
                            const C *tape2, int ncu) {
     const Lattice &L = c->lat;
-    using T = Tile3<C, C>;
+    using T = Tile3A<C>;
     int nblocks;
     Geo2 g =
         make_geo2(L, T::X, T::Y, three_step_zc(with_cus(L, ncu), T::X, T::Y, T::ZC), &nblocks);
