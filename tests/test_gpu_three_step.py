@@ -49,6 +49,34 @@ def test_three_step_pass_equals_single_steps(gpu, kind, shifted, shape):
 
 
 @pytest.mark.parametrize("kind", KINDS3)
+def test_full_rounds_and_a_tail_equal_single_steps(gpu, kind):
+    """320x520x40 on 256 CUs: the three-step launch has 1040 workgroups
+    (four rounds of 256 and a tail of 16: 64x10 tiles in 10-plane chunks,
+    FP64 compute 32x10 in 20-plane chunks) and the two-step launch 1248
+    (128x5, four rounds and 224) or, in FP64 compute, 2080 (64x5, eight
+    rounds and 32), so the block order maps full rounds and spreads a tail
+    over the XCDs in one launch.  Twelve steps of each pass depth equal
+    twelve single steps."""
+    say("full rounds and a tail 320x520x40", kind)
+    shape = (320, 520, 40)
+    u = bgk.hit_velocity(*shape, 4.0, 0.02, 31)
+    ref = bgk.Lattice(*shape, omega=1.95, storage=kind, two_step=False)
+    ref.init_equilibrium(u)
+    for _ in range(12):
+        ref.col_MC()
+    want = ref.checksum()
+    ref.close()
+    for three in (True, False):
+        a = bgk.Lattice(*shape, omega=1.95, storage=kind, three_step=three, timing=True)
+        a.init_equilibrium(u)
+        a.step(12)
+        t = a.timing_passes()
+        assert t[3 if three else 2][1] == (4 if three else 6), t
+        assert a.checksum() == want, three
+        a.close()
+
+
+@pytest.mark.parametrize("kind", KINDS3)
 def test_three_step_equals_two_step_passes(gpu, kind):
     """Three-step passes against two-step passes (BGK_FLAG_NO_THREE_STEP) on
     a lattice where both engage by themselves: 24 steps, checksums equal."""
