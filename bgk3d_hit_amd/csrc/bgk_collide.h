@@ -863,6 +863,23 @@ inline int three_step_zc(const Lattice &L, int tx, int ty, int zc_max) {
     return best;
 }
 
+// The grown-tile row that main wave w of k_collide3 computes.  With
+// TX = 64 a wave is one row, and waves w, w+4, ... issue on one SIMD, so a
+// stage takes the time of its busiest SIMD's waves.  The second stage's
+// rows 1 and EY-2 and the edge wave (wave EY) compute only part of their
+// sites' populations (s2kind); rows EY-2 and EY-4 swap waves so that row
+// EY-2 shares the edge wave's SIMD: 64x10 tiles then give the second
+// stage's busiest SIMD two full collisions and two partial ones, against
+// three full and the edge wave's partial one in row order; the third
+// stage keeps at most three collisions per SIMD.
+template <int TX, int EY>
+__device__ constexpr int k3_row(int w) {
+    if constexpr (TX == 64 && EY >= 8)
+        return w == EY - 4 ? EY - 2 : (w == EY - 2 ? EY - 4 : w);
+    else
+        return w;
+}
+
 // SLAB: the slab's bulk (z offset g.zoff, no wrap), compiled apart so that
 // the periodic pass carries none of it.  FACES (periodic lattice): also write
 // the face images of the first step's state to g.faces and of the second
@@ -884,6 +901,8 @@ __global__ __launch_bounds__(collide3_threads(TX, TY), 1) void k_collide3(Pops<S
                                                                           Pops<S> dst, Geo2 g,
                                                                           C omega, C cte1) {
     static_assert(!(RED && (SLAB || FACES)), "RED runs on the periodic lattice alone");
+    // the second stage trimmed to what the third reads (s2kind, k3_row)
+    constexpr bool TRIM2 = sizeof(S) == 2 && !FACES && !RED;
     constexpr int EX = TX + 4, EY = TY + 4;
     constexpr int NMAIN = TX * EY;
     static_assert(4 * EY <= 64, "the edge wave covers four grown columns");
@@ -904,7 +923,7 @@ __global__ __launch_bounds__(collide3_threads(TX, TY), 1) void k_collide3(Pops<S
     bool active;
     if (tid < NMAIN) {
         ex = tid % TX + 2;
-        ey = tid / TX;
+        ey = TRIM2 ? k3_row<TX, EY>(tid / TX) : tid / TX;
         active = true;
     } else {
         const int e = tid - NMAIN;
@@ -918,6 +937,16 @@ __global__ __launch_bounds__(collide3_threads(TX, TY), 1) void k_collide3(Pops<S
     const int gy = ((ly % g.ny) + g.ny) % g.ny;
     const bool mid = active && ex >= 1 && ex <= EX - 2 && ey >= 1 && ey <= EY - 2;
     const bool owner = tid < NMAIN && ey >= 2 && ey <= EY - 3 && lx < g.nx && ly < g.ny;
+    // What the third stage reads of a second-stage site: all of it at the
+    // own tile's sites; at row 1 only the populations with c_y = +1, at row
+    // EY-2 those with c_y = -1, at the edge columns 1 and EX-2 (the edge
+    // wave) those with c_x != 0.  The others are left uncomputed (the
+    // compiler drops their equilibrium and relaxation); the kind is uniform
+    // over the second stage's lanes of each wave.
+    // Taken for 16-bit storage (the collisions bound the pass; 107 and 124
+    // VGPRs): with 4- and 8-byte storage the extra code spilled registers.
+    const int s2kind =
+        !TRIM2 ? 0 : (tid >= NMAIN ? 3 : (ey == 1 ? 1 : (ey == EY - 2 ? 2 : 0)));
 
     const int xm = (gx == 0) ? g.nx - 1 : gx - 1;
     const int xp = (gx == g.nx - 1) ? 0 : gx + 1;
@@ -1017,15 +1046,30 @@ __global__ __launch_bounds__(collide3_threads(TX, TY), 1) void k_collide3(Pops<S
         lds_barrier();
         const bool do2 = mid && it >= 2;
         if (do2) {  // S2 at plane z-1
-            C f2[19], b[19], ux, uy, uz;
+            C f2[19];
 #pragma unroll
             for (int q = 0; q < 19; q++)
                 f2[q] = Storage<S>::template load<C>(pull(l1, q, ey, ex, pb, s1[QN], r1O_2, r1R_1));
-            collide_site<C, SH>(f2, b, omega, cte1, ux, uy, uz);
-            store_all<S, C>(b, s2);
+            C ux, uy, uz;
+            const Moments<C> mo = collide_moments<C, SH>(f2, cte1, ux, uy, uz);
+            auto second = [&](auto sel) {
+                constexpr uint32_t M = decltype(sel)::value;
+                C b[19];
+                collide_relax<M>(f2, mo, omega, b);
+                store_sel<M, S, C>(b, s2);
 #pragma unroll
-            for (int q = 0; q < 19; q++)
-                if (pgroup<1>(q) == 0) l2[pbase(0) + pslot<1>(q)][ey - 1][ex - 1] = s2[q];
+                for (int q = 0; q < 19; q++)
+                    if (((M >> q) & 1u) && pgroup<1>(q) == 0)
+                        l2[pbase(0) + pslot<1>(q)][ey - 1][ex - 1] = s2[q];
+            };
+            if (s2kind == 0)
+                second(std::integral_constant<uint32_t, ALL_POPS>{});
+            else if (s2kind == 1)
+                second(std::integral_constant<uint32_t, pops_cy(1)>{});
+            else if (s2kind == 2)
+                second(std::integral_constant<uint32_t, pops_cy(-1)>{});
+            else
+                second(std::integral_constant<uint32_t, ALL_POPS & ~pops_cx(0)>{});
             if (FACES && owner && z - 1 >= za && z - 1 < zb)
                 write_faces<S, true>(g.faces2, g, s2, gx, gy, wrapz(z - 1));
             if constexpr (RED) {
@@ -1080,15 +1124,27 @@ __global__ __launch_bounds__(collide3_threads(TX, TY), 1) void k_collide3(Pops<S
         if (more) load_part(z + 1, nxt, qB, q19);
         lds_barrier();
         if (do2) {
+            auto late = [&](auto sel) {
+                constexpr uint32_t M = decltype(sel)::value;
 #pragma unroll
-            for (int q = 0; q < 19; q++) {
-                if (pgroup<1>(q) == 1) l2[pbase(1) + pslot<1>(q)][ey - 1][ex - 1] = s2[q];
-                if (pgroup<1>(q) == 2) l2[pb + pslot<1>(q)][ey - 1][ex - 1] = s2[q];
+                for (int q = 0; q < 19; q++) {
+                    if (!((M >> q) & 1u)) continue;
+                    if (pgroup<1>(q) == 1) l2[pbase(1) + pslot<1>(q)][ey - 1][ex - 1] = s2[q];
+                    if (pgroup<1>(q) == 2) l2[pb + pslot<1>(q)][ey - 1][ex - 1] = s2[q];
+                }
+            };
+            if (s2kind == 0) {
+                late(std::integral_constant<uint32_t, ALL_POPS>{});
+                r2O_2 = r2O_1;  // the column's z-only populations (own sites)
+                r2O_1 = s2[QO];
+                r2R_1 = s2[QR];
+            } else if (s2kind == 1) {
+                late(std::integral_constant<uint32_t, pops_cy(1)>{});
+            } else if (s2kind == 2) {
+                late(std::integral_constant<uint32_t, pops_cy(-1)>{});
+            } else {
+                late(std::integral_constant<uint32_t, ALL_POPS & ~pops_cx(0)>{});
             }
-
-            r2O_2 = r2O_1;
-            r2O_1 = s2[QO];
-            r2R_1 = s2[QR];
         }
         r1O_2 = r1O_1;
         r1O_1 = s1[QO];

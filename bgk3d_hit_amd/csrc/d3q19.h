@@ -113,12 +113,16 @@ struct Storage<bf16_t> {
 // The conversions are written as asm: the compiler lowered the same pair
 // conversion written as a two-element vector conversion to one converted
 // element stored twice, and the asm also keeps the f32 inputs opaque (see
-// Storage<half_t>::store).
-template <typename S, typename C>
-__device__ __forceinline__ void store_all(const C (&b)[19], S (&out)[19]) {
+// Storage<half_t>::store).  store_sel<M> converts only the populations in
+// the bit mask M (a pair if either of its two is in M) and leaves the other
+// elements of out[] unset; the values it does convert are those of
+// store_all.
+template <uint32_t M, typename S, typename C>
+__device__ __forceinline__ void store_sel(const C (&b)[19], S (&out)[19]) {
     if constexpr (std::is_same<S, half_t>::value || std::is_same<S, bf16_t>::value) {
 #pragma unroll
         for (int q = 0; q + 1 < 19; q += 2) {
+            if (!((M >> q) & 3u)) continue;
             const float lo = (float)b[q], hi = (float)b[q + 1];
             uint32_t p;
             if constexpr (std::is_same<S, half_t>::value)
@@ -128,11 +132,30 @@ __device__ __forceinline__ void store_all(const C (&b)[19], S (&out)[19]) {
             out[q].bits = (uint16_t)p;
             out[q + 1].bits = (uint16_t)(p >> 16);
         }
-        out[18] = Storage<S>::template store<C>(b[18]);
+        if ((M >> 18) & 1u) out[18] = Storage<S>::template store<C>(b[18]);
     } else {
 #pragma unroll
-        for (int q = 0; q < 19; q++) out[q] = Storage<S>::template store<C>(b[q]);
+        for (int q = 0; q < 19; q++)
+            if ((M >> q) & 1u) out[q] = Storage<S>::template store<C>(b[q]);
     }
+}
+constexpr uint32_t ALL_POPS = (1u << 19) - 1;
+template <typename S, typename C>
+__device__ __forceinline__ void store_all(const C (&b)[19], S (&out)[19]) {
+    store_sel<ALL_POPS, S, C>(b, out);
+}
+// the populations whose c_x (c_y) is s, as a bit mask
+__host__ __device__ constexpr uint32_t pops_cx(int s) {
+    uint32_t m = 0;
+    for (int q = 0; q < 19; q++)
+        if (cx(q) == s) m |= 1u << q;
+    return m;
+}
+__host__ __device__ constexpr uint32_t pops_cy(int s) {
+    uint32_t m = 0;
+    for (int q = 0; q < 19; q++)
+        if (cy(q) == s) m |= 1u << q;
+    return m;
 }
 
 // ---------------------------------------------------------------- collide
@@ -185,7 +208,21 @@ __device__ __forceinline__ C add_cte0(C cte0, C y) {
         return cte0 + y;
 }
 
-// One site of SRC/col_MC.F90:133-219,333-351.  x[] are the pulled
+// What the equilibria of a site need once its moments are known: the
+// linear terms rf * (velocity combinations), the quadratic terms and the
+// density products (SRC/col_MC.F90:133-193).
+template <typename C>
+struct Moments {
+    C vx, vy, vz, vxpy, vxmy, vxpz, vxmz, vypz, vymz;  // times rf
+    C qxpy, qxmy, qxpz, qxmz, qypz, qymz, qx, qy, qz, q0;
+    C rp0, rp1, rp2, s0, s1, s2;
+};
+
+// One site of SRC/col_MC.F90:133-219,333-351, in two parts: collide_moments
+// (the density, the velocity and the terms above) and collide_relax<M>
+// (the equilibria e_q and the relaxed b_q = x_q - omega (x_q - e_q) of the
+// populations in the bit mask M, the others of b[] left unset).
+// collide_site is both for all populations.  x[] are the pulled
 // populations, b[] receives the post-collision populations (b[18] is the
 // rest population the reference updates in place).  cte1 = 1 for shifted
 // storage (SRC/col_MC.F90:86-91).
@@ -196,8 +233,8 @@ __device__ __forceinline__ C add_cte0(C cte0, C y) {
 // additions of cte0 = +0; results are bit-identical to SH = false with
 // cte1 = 1.
 template <typename C, bool SH = false>
-__device__ __forceinline__ void collide_site(const C (&x)[19], C (&b)[19], C omega, C cte1_arg,
-                                             C &ux, C &uy, C &uz) {
+__device__ __forceinline__ Moments<C> collide_moments(const C (&x)[19], C cte1_arg, C &ux, C &uy,
+                                                      C &uz) {
     using k = K<C>;
     const C cte1 = SH ? k::uno : cte1_arg;
     const C cte0 = k::uno - cte1;
@@ -215,76 +252,78 @@ __device__ __forceinline__ void collide_site(const C (&x)[19], C (&b)[19], C ome
     ux = vx;
     uy = vy;
     uz = vz;
+    Moments<C> m;
     const C vx2 = vx * vx, vy2 = vy * vy, vz2 = vz * vz;
     const C vsq = vx2 + vy2 + vz2;
     C vxpy = vx + vy;
-    const C qxpy = add_cte0<SH>(cte0, k::qf * (k::tre * vxpy * vxpy - vsq));
+    m.qxpy = add_cte0<SH>(cte0, k::qf * (k::tre * vxpy * vxpy - vsq));
     C vxmy = vx - vy;
-    const C qxmy = add_cte0<SH>(cte0, k::qf * (k::tre * vxmy * vxmy - vsq));
+    m.qxmy = add_cte0<SH>(cte0, k::qf * (k::tre * vxmy * vxmy - vsq));
     C vxpz = vx + vz;
-    const C qxpz = add_cte0<SH>(cte0, k::qf * (k::tre * vxpz * vxpz - vsq));
+    m.qxpz = add_cte0<SH>(cte0, k::qf * (k::tre * vxpz * vxpz - vsq));
     C vxmz = vx - vz;
-    const C qxmz = add_cte0<SH>(cte0, k::qf * (k::tre * vxmz * vxmz - vsq));
+    m.qxmz = add_cte0<SH>(cte0, k::qf * (k::tre * vxmz * vxmz - vsq));
     C vypz = vy + vz;
-    const C qypz = add_cte0<SH>(cte0, k::qf * (k::tre * vypz * vypz - vsq));
+    m.qypz = add_cte0<SH>(cte0, k::qf * (k::tre * vypz * vypz - vsq));
     C vymz = vy - vz;
-    const C qymz = add_cte0<SH>(cte0, k::qf * (k::tre * vymz * vymz - vsq));
-    const C qx = add_cte0<SH>(cte0, k::qf * (k::tre * vx2 - vsq));
-    const C qy = add_cte0<SH>(cte0, k::qf * (k::tre * vy2 - vsq));
-    const C qz = add_cte0<SH>(cte0, k::qf * (k::tre * vz2 - vsq));
-    const C q0 = cte0 + k::qf * (-vsq);  // qf * (-vsq) is -0 when vsq = 0: kept
+    m.qymz = add_cte0<SH>(cte0, k::qf * (k::tre * vymz * vymz - vsq));
+    m.qx = add_cte0<SH>(cte0, k::qf * (k::tre * vx2 - vsq));
+    m.qy = add_cte0<SH>(cte0, k::qf * (k::tre * vy2 - vsq));
+    m.qz = add_cte0<SH>(cte0, k::qf * (k::tre * vz2 - vsq));
+    m.q0 = cte0 + k::qf * (-vsq);  // qf * (-vsq) is -0 when vsq = 0: kept
 
-    vx = k::rf * vx;
-    vy = k::rf * vy;
-    vz = k::rf * vz;
-    vxpy = k::rf * vxpy;
-    vxmy = k::rf * vxmy;
-    vxpz = k::rf * vxpz;
-    vxmz = k::rf * vxmz;
-    vypz = k::rf * vypz;
-    vymz = k::rf * vymz;
+    m.vx = k::rf * vx;
+    m.vy = k::rf * vy;
+    m.vz = k::rf * vz;
+    m.vxpy = k::rf * vxpy;
+    m.vxmy = k::rf * vxmy;
+    m.vxpz = k::rf * vxpz;
+    m.vxmz = k::rf * vxmz;
+    m.vypz = k::rf * vypz;
+    m.vymz = k::rf * vymz;
 
-    const C rp0 = rho * k::p0, rp1 = rho * k::p1, rp2 = rho * k::p2;
-    const C s2 = cte1 * (rp2 - k::p2), s1 = cte1 * (rp1 - k::p1), s0 = cte1 * (rp0 - k::p0);
-    const C e01 = rp2 * (+vxmy + qxmy) + s2;
-    const C e02 = rp2 * (+vxmz + qxmz) + s2;
-    const C e03 = rp2 * (+vxpy + qxpy) + s2;
-    const C e04 = rp2 * (+vxpz + qxpz) + s2;
-    const C e05 = rp1 * (+vx + qx) + s1;
-    const C e06 = rp1 * (+vz + qz) + s1;
-    const C e07 = rp2 * (+vypz + qypz) + s2;
-    const C e08 = rp1 * (+vy + qy) + s1;
-    const C e09 = rp2 * (+vymz + qymz) + s2;
-    const C e10 = rp2 * (-vxpy + qxpy) + s2;
-    const C e11 = rp2 * (-vxpz + qxpz) + s2;
-    const C e12 = rp2 * (-vxmy + qxmy) + s2;
-    const C e13 = rp2 * (-vxmz + qxmz) + s2;
-    const C e14 = rp1 * (-vx + qx) + s1;
-    const C e15 = rp1 * (-vz + qz) + s1;
-    const C e16 = rp2 * (-vypz + qypz) + s2;
-    const C e17 = rp1 * (-vy + qy) + s1;
-    const C e18 = rp2 * (-vymz + qymz) + s2;
-    const C e19 = rp0 * (+q0) + s0;
+    m.rp0 = rho * k::p0;
+    m.rp1 = rho * k::p1;
+    m.rp2 = rho * k::p2;
+    m.s2 = cte1 * (m.rp2 - k::p2);
+    m.s1 = cte1 * (m.rp1 - k::p1);
+    m.s0 = cte1 * (m.rp0 - k::p0);
+    return m;
+}
 
-    b[0] = x01 - omega * (x01 - e01);
-    b[1] = x02 - omega * (x02 - e02);
-    b[2] = x03 - omega * (x03 - e03);
-    b[3] = x04 - omega * (x04 - e04);
-    b[4] = x05 - omega * (x05 - e05);
-    b[5] = x06 - omega * (x06 - e06);
-    b[6] = x07 - omega * (x07 - e07);
-    b[7] = x08 - omega * (x08 - e08);
-    b[8] = x09 - omega * (x09 - e09);
-    b[9] = x10 - omega * (x10 - e10);
-    b[10] = x11 - omega * (x11 - e11);
-    b[11] = x12 - omega * (x12 - e12);
-    b[12] = x13 - omega * (x13 - e13);
-    b[13] = x14 - omega * (x14 - e14);
-    b[14] = x15 - omega * (x15 - e15);
-    b[15] = x16 - omega * (x16 - e16);
-    b[16] = x17 - omega * (x17 - e17);
-    b[17] = x18 - omega * (x18 - e18);
-    b[18] = x19 - omega * (x19 - e19);
+template <uint32_t M, typename C>
+__device__ __forceinline__ void collide_relax(const C (&x)[19], const Moments<C> &m, C omega,
+                                              C (&b)[19]) {
+    C e[19];
+    if (M & (1u << 0)) e[0] = m.rp2 * (+m.vxmy + m.qxmy) + m.s2;
+    if (M & (1u << 1)) e[1] = m.rp2 * (+m.vxmz + m.qxmz) + m.s2;
+    if (M & (1u << 2)) e[2] = m.rp2 * (+m.vxpy + m.qxpy) + m.s2;
+    if (M & (1u << 3)) e[3] = m.rp2 * (+m.vxpz + m.qxpz) + m.s2;
+    if (M & (1u << 4)) e[4] = m.rp1 * (+m.vx + m.qx) + m.s1;
+    if (M & (1u << 5)) e[5] = m.rp1 * (+m.vz + m.qz) + m.s1;
+    if (M & (1u << 6)) e[6] = m.rp2 * (+m.vypz + m.qypz) + m.s2;
+    if (M & (1u << 7)) e[7] = m.rp1 * (+m.vy + m.qy) + m.s1;
+    if (M & (1u << 8)) e[8] = m.rp2 * (+m.vymz + m.qymz) + m.s2;
+    if (M & (1u << 9)) e[9] = m.rp2 * (-m.vxpy + m.qxpy) + m.s2;
+    if (M & (1u << 10)) e[10] = m.rp2 * (-m.vxpz + m.qxpz) + m.s2;
+    if (M & (1u << 11)) e[11] = m.rp2 * (-m.vxmy + m.qxmy) + m.s2;
+    if (M & (1u << 12)) e[12] = m.rp2 * (-m.vxmz + m.qxmz) + m.s2;
+    if (M & (1u << 13)) e[13] = m.rp1 * (-m.vx + m.qx) + m.s1;
+    if (M & (1u << 14)) e[14] = m.rp1 * (-m.vz + m.qz) + m.s1;
+    if (M & (1u << 15)) e[15] = m.rp2 * (-m.vypz + m.qypz) + m.s2;
+    if (M & (1u << 16)) e[16] = m.rp1 * (-m.vy + m.qy) + m.s1;
+    if (M & (1u << 17)) e[17] = m.rp2 * (-m.vymz + m.qymz) + m.s2;
+    if (M & (1u << 18)) e[18] = m.rp0 * (+m.q0) + m.s0;
+#pragma unroll
+    for (int q = 0; q < 19; q++)
+        if ((M >> q) & 1u) b[q] = x[q] - omega * (x[q] - e[q]);
+}
+
+template <typename C, bool SH = false>
+__device__ __forceinline__ void collide_site(const C (&x)[19], C (&b)[19], C omega, C cte1_arg,
+                                             C &ux, C &uy, C &uz) {
+    const Moments<C> m = collide_moments<C, SH>(x, cte1_arg, ux, uy, uz);
+    collide_relax<(1u << 19) - 1>(x, m, omega, b);
 }
 
 // At-site moments of SRC/diagno.F90:96-111 (also the first lines of
