@@ -160,22 +160,25 @@ def test_driver_two_step_pairs_match_single_steps(gpu, tmp_path):
 
 
 @pytest.mark.gpu
-def test_driver_refcompat_fused_passes(gpu, tmp_path):
+@pytest.mark.parametrize("storage", ["f32", "f16", "bf16"])
+def test_driver_refcompat_fused_passes(gpu, tmp_path, storage):
     """The reference-compatible diagnostics (stale halo images) with the
     driver's fused passes (7 steps between outputs: a three-step pass and two
     two-step passes) on a 256^3 lattice: equal to the reference call order
-    (boundaries + col_MC every step) exactly.  The task log's launch counts
-    by pass depth show three-step passes and add up to itfin steps."""
+    (boundaries + col_MC every step) exactly, for FP32 and the 16-bit kinds
+    (whose three-step passes trim their second stage).  The task log's
+    launch counts by pass depth show three-step passes and add up to itfin
+    steps."""
     n, omega, itfin, icheck = 256, 1.95, 21, 7
     svisc = (2.0 / omega - 1.0) / 6.0
     deck = (f"&parameters\nlx={n}\nly={n}\nlz={n}\nomega={omega}\nitfin={itfin}\n"
             f"icheck={icheck}\ninit='hit'\nk0=4.0\nurms=0.02\nseed=99\n"
-            f"storage='f32'\ndiag='refcompat'\n/\n")
-    print("[gpu-case] fortran driver 256^3 f32 refcompat fused passes", flush=True)
+            f"storage='{storage}'\ndiag='refcompat'\n/\n")
+    print(f"[gpu-case] fortran driver 256^3 {storage} refcompat fused passes", flush=True)
     run_driver(tmp_path, deck)
     dg = np.loadtxt(tmp_path / "diagno.full.dat", ndmin=2)
     ds = np.loadtxt(tmp_path / "dissipation.full.dat", ndmin=2)
-    lat = bgk.Lattice(n, n, n, omega=omega, storage="f32", refcompat_halos=True,
+    lat = bgk.Lattice(n, n, n, omega=omega, storage=storage, refcompat_halos=True,
                       two_step=False)
     lat.init_hit(4.0, 0.02, 99)
     row = 0
