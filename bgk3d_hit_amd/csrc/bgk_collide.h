@@ -944,7 +944,9 @@ __global__ __launch_bounds__(collide3_threads(TX, TY), 1) void k_collide3(Pops<S
     // compiler drops their equilibrium and relaxation); the kind is uniform
     // over the second stage's lanes of each wave.
     // Taken for 16-bit storage (the collisions bound the pass; 107 and 124
-    // VGPRs): with 4- and 8-byte storage the extra code spilled registers.
+    // VGPRs).  FP32 storage spilled registers with it (and trimming the edge
+    // wave alone measured slower); FP64 compute is bound by latency, not by
+    // its collisions.
     const int s2kind =
         !TRIM2 ? 0 : (tid >= NMAIN ? 3 : (ey == 1 ? 1 : (ey == EY - 2 ? 2 : 0)));
 
