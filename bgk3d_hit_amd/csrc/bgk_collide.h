@@ -393,6 +393,9 @@ struct Geo2 {
     long long ystride, zstride;  // element pitches of rows and planes (as Geo)
     unsigned pstride;            // BUF: bytes from population q to q+1
     int xcd_group;               // workgroups per XCD in one round (CUs / 8)
+    int full_cols;  // k_collide3: the first full_cols workgroups each take a
+                    // whole tile column (all of [zlo, zhi)); the rest of the
+                    // columns go in chunks of zc planes (0: all in chunks)
     void *faces;   // FACES: face images of the first step (k_pack_faces layout)
     void *faces2;  // FACES, three-step pass: face images of the second step
     double *red;   // RED: [2][gridDim.x][RED_NV] partial sums (diagno, dissipation)
@@ -416,6 +419,7 @@ static Geo2 make_geo2(const Lattice &L, int tx, int ty, int zc, int *nblocks) {
     g.zstride = L.zstride;
     g.pstride = (unsigned)((const char *)L.pop[0][1] - (const char *)L.pop[0][0]);
     g.xcd_group = L.ncu >= 8 ? L.ncu / 8 : 1;
+    g.full_cols = 0;
     g.faces = nullptr;
     g.faces2 = nullptr;
     g.red = nullptr;
@@ -863,6 +867,53 @@ inline int three_step_zc(const Lattice &L, int tx, int ty, int zc_max) {
     return best;
 }
 
+// Whole tile columns first in a periodic three-step pass (Geo2::full_cols):
+// a chunk of zc planes costs zc + 4 plane iterations (the first two steps'
+// halo planes), so on a lattice with c >= ncu tile columns, k rounds of
+// whole columns (k ncu workgroups of nz + 4 iterations) and the other
+// c - k ncu columns in chunks can take fewer plane iterations per CU than
+// all of them in chunks; the chunks follow the model of three_step_zc.
+// 512^3, 64x10 tiles, 256 CUs: 256 whole columns and 160 columns in
+// 64-plane chunks, 516 + 5 x 68 = 856 iterations against 13 x 68 = 884.
+struct Split3 {
+    int full_cols, zc, nblocks;
+};
+inline Split3 three_step_split(const Lattice &L, int tx, int ty, int zc_max) {
+    const int zc0 = three_step_zc(L, tx, ty, zc_max);
+    const long long c = (long long)((L.nx + tx - 1) / tx) * ((L.ny + ty - 1) / ty);
+    const int nz = L.nz;
+    Split3 best{0, zc0, zc0 > 0 ? (int)(c * ((nz + zc0 - 1) / zc0)) : 0};
+    const long long ncu = L.ncu > 0 ? L.ncu : 1;
+    if (zc0 <= 0 || L.layout != BGK_LAYOUT_PERIODIC || c < ncu) return best;
+    const int lo = nz < TWO_STEP_MIN_ZC ? nz : TWO_STEP_MIN_ZC;
+    const int hi = nz < zc_max ? nz : zc_max;
+    auto chunked = [&](long long cols, int *zc_out) {
+        long long bc = -1;
+        for (int zc = lo; zc <= hi; zc++) {
+            const long long n = cols * ((nz + zc - 1) / zc);
+            const long long cost = ((n + ncu - 1) / ncu) * (zc + 4);
+            if (bc < 0 || cost <= bc) {
+                bc = cost;
+                *zc_out = zc;
+            }
+        }
+        return bc;
+    };
+    int zr = zc0;
+    long long best_cost = chunked(c, &zr);
+    for (long long k = 1; k * ncu <= c; k++) {
+        const long long rem = c - k * ncu;
+        long long cost = k * (nz + 4);
+        zr = zc0;
+        if (rem > 0) cost += chunked(rem, &zr);
+        if (cost < best_cost) {
+            best_cost = cost;
+            best = Split3{(int)(k * ncu), zr, (int)(k * ncu + rem * ((nz + zr - 1) / zr))};
+        }
+    }
+    return best;
+}
+
 // The grown-tile row that main wave w of k_collide3 computes.  With
 // TX = 64 a wave is one row, and waves w, w+4, ... issue on one SIMD, so a
 // stage takes the time of its busiest SIMD's waves.  The second stage's
@@ -911,12 +962,25 @@ __global__ __launch_bounds__(collide3_threads(TX, TY), 1) void k_collide3(Pops<S
     __shared__ S l1[20][EY][EX];             // S1, site (ey, ex)
     __shared__ S l2[20][EY - 2][EX - 2];     // S2, site (ey, ex) at [ey-1][ex-1]
 
-    const int t = xcd_round_remap((int)blockIdx.x, (int)gridDim.x, g.xcd_group);
-    const int tx = t % g.tiles_x;
-    const int r = t / g.tiles_x;
-    const int ty = r % g.tiles_y;
-    const int za = (SLAB ? g.zlo : 0) + (r / g.tiles_y) * g.zc;
-    const int zb = min(SLAB ? g.zhi : g.nz, za + g.zc);
+    // tile column and z range: whole columns first (g.full_cols workgroups,
+    // whole rounds), then the other columns in chunks of g.zc planes, each
+    // part in the round block order
+    const int b = (int)blockIdx.x, nb = (int)gridDim.x;
+    const int zlo = SLAB ? g.zlo : 0, zhi = SLAB ? g.zhi : g.nz;
+    int col, za, zb;
+    if (b < g.full_cols) {
+        col = xcd_round_remap(b, g.full_cols, g.xcd_group);
+        za = zlo;
+        zb = zhi;
+    } else {
+        const int rc = g.tiles_x * g.tiles_y - g.full_cols;
+        const int t = xcd_round_remap(b - g.full_cols, nb - g.full_cols, g.xcd_group);
+        col = g.full_cols + t % rc;
+        za = zlo + (t / rc) * g.zc;
+        zb = min(zhi, za + g.zc);
+    }
+    const int tx = col % g.tiles_x;
+    const int ty = col / g.tiles_x;
 
     const int tid = (int)threadIdx.x;
     int ex, ey;

@@ -86,6 +86,12 @@ static hipError_t collide3_sh(const Lattice &L, int src, double omega, void *fac
     if (zc <= 0) return hipErrorInvalidValue;  // caller checks three_step_ok first
     int nblocks;
     Geo2 g = make_geo2(L, T::X, T::Y, zc, &nblocks);
+    if (g.periodic_z) {  // whole tile columns first where that takes fewer iterations
+        const Split3 sp = three_step_split(L, T::X, T::Y, T::ZC);
+        g.full_cols = sp.full_cols;
+        g.zc = sp.zc;
+        nblocks = sp.nblocks;
+    }
     const C w = (C)omega, cte1 = (C)(L.shifted ? 1.0 : 0.0);
     if (faces1 || faces2) {
         if (!g.periodic_z || !faces1 || !faces2) return hipErrorInvalidValue;

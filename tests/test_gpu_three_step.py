@@ -50,13 +50,14 @@ def test_three_step_pass_equals_single_steps(gpu, kind, shifted, shape):
 
 @pytest.mark.parametrize("kind", KINDS3)
 def test_full_rounds_and_a_tail_equal_single_steps(gpu, kind):
-    """320x520x40 on 256 CUs: the three-step launch has 1040 workgroups
-    (four rounds of 256 and a tail of 16: 64x10 tiles in 10-plane chunks,
-    FP64 compute 32x10 in 20-plane chunks) and the two-step launch 1248
-    (128x5, four rounds and 224) or, in FP64 compute, 2080 (64x5, eight
-    rounds and 32), so the block order maps full rounds and spreads a tail
-    over the XCDs in one launch.  Twelve steps of each pass depth equal
-    twelve single steps."""
+    """320x520x40 on 256 CUs: the three-step launch takes a round of whole
+    tile columns and the rest in chunks (64x10 tiles: 256 columns whole and
+    4 in 8-plane chunks, 276 workgroups; FP64 compute, 32x10: 512 whole and
+    8 in chunks, 552) and the two-step launch 1248 workgroups (128x5, four
+    rounds and a tail of 224) or, in FP64 compute, 2080 (64x5, eight rounds
+    and 32), so the block order maps full rounds and spreads a tail over
+    the XCDs in one launch.  Twelve steps of each pass depth equal twelve
+    single steps."""
     say("full rounds and a tail 320x520x40", kind)
     shape = (320, 520, 40)
     u = bgk.hit_velocity(*shape, 4.0, 0.02, 31)
