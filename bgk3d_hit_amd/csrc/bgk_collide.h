@@ -867,7 +867,7 @@ inline int three_step_zc(const Lattice &L, int tx, int ty, int zc_max) {
     return best;
 }
 
-// Whole tile columns first in a periodic three-step pass (Geo2::full_cols):
+// Whole tile columns first in a three-step pass (Geo2::full_cols):
 // a chunk of zc planes costs zc + 4 plane iterations (the first two steps'
 // halo planes), so on a lattice with c >= ncu tile columns, k rounds of
 // whole columns (k ncu workgroups of nz + 4 iterations) and the other
@@ -881,10 +881,11 @@ struct Split3 {
 inline Split3 three_step_split(const Lattice &L, int tx, int ty, int zc_max) {
     const int zc0 = three_step_zc(L, tx, ty, zc_max);
     const long long c = (long long)((L.nx + tx - 1) / tx) * ((L.ny + ty - 1) / ty);
-    const int nz = L.nz;
+    // third-step planes: all of a periodic lattice, a slab's bulk 3 .. nz-4
+    const int nz = L.layout == BGK_LAYOUT_PERIODIC ? L.nz : L.nz - 6;
     Split3 best{0, zc0, zc0 > 0 ? (int)(c * ((nz + zc0 - 1) / zc0)) : 0};
     const long long ncu = L.ncu > 0 ? L.ncu : 1;
-    if (zc0 <= 0 || L.layout != BGK_LAYOUT_PERIODIC || c < ncu) return best;
+    if (zc0 <= 0 || c < ncu) return best;
     const int lo = nz < TWO_STEP_MIN_ZC ? nz : TWO_STEP_MIN_ZC;
     const int hi = nz < zc_max ? nz : zc_max;
     auto chunked = [&](long long cols, int *zc_out) {
