@@ -459,8 +459,11 @@ hipError_t launch_tape3_k(bgk_ctx *c, C *tape0, C *tape1, C *tape2, hipStream_t 
     const Lattice &L = c->lat;
     using T = Tile3A<C>;
     int nblocks;
-    const Geo2 g =
-        make_geo2(L, T::X, T::Y, three_step_zc(with_cus(L, ncu), T::X, T::Y, T::ZC), &nblocks);
+    Geo2 g = make_geo2(L, T::X, T::Y, 1, &nblocks);
+    const Split3 sp = three_step_split(with_cus(L, ncu), T::X, T::Y, T::ZC);
+    g.full_cols = sp.full_cols;
+    g.zc = sp.zc;
+    nblocks = sp.nblocks;
     const CollideOp3<C, true, true, KEEP> op{(C)c->omega, (C)1, tape0, tape1, tape2,
                                              (size_t)L.nsite};
     hipLaunchKernelGGL((k_tape3<S, C, T::X, T::Y, KEEP>), dim3(nblocks),
@@ -563,8 +566,11 @@ hipError_t launch_adjoint3(bgk_ctx *c, const C *lin, C *lout, const C *tape0, co
     const Lattice &L = c->lat;
     using T = Tile3A<C>;
     int nblocks;
-    Geo2 g =
-        make_geo2(L, T::X, T::Y, three_step_zc(with_cus(L, ncu), T::X, T::Y, T::ZC), &nblocks);
+    Geo2 g = make_geo2(L, T::X, T::Y, 1, &nblocks);
+    const Split3 sp = three_step_split(with_cus(L, ncu), T::X, T::Y, T::ZC);
+    g.full_cols = sp.full_cols;
+    g.zc = sp.zc;
+    nblocks = sp.nblocks;
     // lambda: the lattice's interleaved layout in the compute kind
     g.pstride = (unsigned)(L.nx * sizeof(C));
     Pops<C> a, b;

@@ -915,6 +915,33 @@ inline Split3 three_step_split(const Lattice &L, int tx, int ty, int zc_max) {
     return best;
 }
 
+// Tile column and z range [za, zb) of this workgroup in a three-step pass
+// over the third-step planes [zlo, zhi): the first g.full_cols workgroups
+// (whole rounds) take whole columns, the rest of the columns go in chunks
+// of g.zc planes; each part in the round block order (xcd_round_remap).
+struct Tile3Pos {
+    int tx, ty, za, zb;
+};
+__device__ __forceinline__ Tile3Pos tile3_pos(const Geo2 &g, int zlo, int zhi) {
+    const int b = (int)blockIdx.x, nb = (int)gridDim.x;
+    int col;
+    Tile3Pos p;
+    if (b < g.full_cols) {
+        col = xcd_round_remap(b, g.full_cols, g.xcd_group);
+        p.za = zlo;
+        p.zb = zhi;
+    } else {
+        const int rc = g.tiles_x * g.tiles_y - g.full_cols;
+        const int t = xcd_round_remap(b - g.full_cols, nb - g.full_cols, g.xcd_group);
+        col = g.full_cols + t % rc;
+        p.za = zlo + (t / rc) * g.zc;
+        p.zb = min(zhi, p.za + g.zc);
+    }
+    p.tx = col % g.tiles_x;
+    p.ty = col / g.tiles_x;
+    return p;
+}
+
 // The grown-tile row that main wave w of k_collide3 computes.  With
 // TX = 64 a wave is one row, and waves w, w+4, ... issue on one SIMD, so a
 // stage takes the time of its busiest SIMD's waves.  The second stage's
@@ -963,25 +990,8 @@ __global__ __launch_bounds__(collide3_threads(TX, TY), 1) void k_collide3(Pops<S
     __shared__ S l1[20][EY][EX];             // S1, site (ey, ex)
     __shared__ S l2[20][EY - 2][EX - 2];     // S2, site (ey, ex) at [ey-1][ex-1]
 
-    // tile column and z range: whole columns first (g.full_cols workgroups,
-    // whole rounds), then the other columns in chunks of g.zc planes, each
-    // part in the round block order
-    const int b = (int)blockIdx.x, nb = (int)gridDim.x;
-    const int zlo = SLAB ? g.zlo : 0, zhi = SLAB ? g.zhi : g.nz;
-    int col, za, zb;
-    if (b < g.full_cols) {
-        col = xcd_round_remap(b, g.full_cols, g.xcd_group);
-        za = zlo;
-        zb = zhi;
-    } else {
-        const int rc = g.tiles_x * g.tiles_y - g.full_cols;
-        const int t = xcd_round_remap(b - g.full_cols, nb - g.full_cols, g.xcd_group);
-        col = g.full_cols + t % rc;
-        za = zlo + (t / rc) * g.zc;
-        zb = min(zhi, za + g.zc);
-    }
-    const int tx = col % g.tiles_x;
-    const int ty = col / g.tiles_x;
+    const Tile3Pos pos = tile3_pos(g, SLAB ? g.zlo : 0, SLAB ? g.zhi : g.nz);
+    const int tx = pos.tx, ty = pos.ty, za = pos.za, zb = pos.zb;
 
     const int tid = (int)threadIdx.x;
     int ex, ey;
@@ -1312,12 +1322,8 @@ __device__ __forceinline__ void three_step_pass(const Pops<S> &src, const Pops<S
     __shared__ S l1[20][EY][EX];          // S1, site (ey, ex)
     __shared__ S l2[20][EY - 2][EX - 2];  // S2, site (ey, ex) at [ey-1][ex-1]
 
-    const int t = xcd_round_remap((int)blockIdx.x, (int)gridDim.x, g.xcd_group);
-    const int tx = t % g.tiles_x;
-    const int r = t / g.tiles_x;
-    const int ty = r % g.tiles_y;
-    const int za = (r / g.tiles_y) * g.zc;
-    const int zb = min(g.nz, za + g.zc);
+    const Tile3Pos pos = tile3_pos(g, 0, g.nz);
+    const int tx = pos.tx, ty = pos.ty, za = pos.za, zb = pos.zb;
 
     const int tid = (int)threadIdx.x;
     int ex, ey;
