@@ -302,6 +302,34 @@ def test_refcompat_images_with_fused_passes(gpu, kind, three_step, shape):
     assert depth[2][1] > 0, depth
 
 
+def test_refcompat_images_512cubed_whole_columns(gpu):
+    """512^3 FP32 with reference halo images: six steps are two three-step
+    passes over whole tile columns and chunks, the second writing the face
+    images; the refcompat dissipation and diagnostics then equal those of
+    the reference call order (boundaries + col_MC every step), as do the
+    populations (checksums)."""
+    say("refcompat 512^3 whole columns")
+    n, omega = 512, 1.95
+    nu = bgk.svisc_from_omega(omega)
+    a = bgk.Lattice(n, n, n, omega=omega, storage="f32", refcompat_halos=True, timing=True)
+    a.init_hit(4.0, 0.02, 5)
+    a.step(6)
+    assert a.timing_passes()[3][1] == 2
+    da = a.dissipation(nu, refcompat=True)
+    ga = a.diagno(refcompat=True)
+    ca = a.checksum()
+    a.close()
+    b = bgk.Lattice(n, n, n, omega=omega, storage="f32", refcompat_halos=True, two_step=False)
+    b.init_hit(4.0, 0.02, 5)
+    for _ in range(6):
+        b.boundaries()
+        b.col_MC()
+    assert b.checksum() == ca
+    assert np.array_equal(b.dissipation(nu, refcompat=True), da)
+    assert np.array_equal(b.diagno(refcompat=True), ga)
+    b.close()
+
+
 def test_refcompat_dissipation_needs_flag(gpu):
     g = bgk.Lattice(8, 8, 8, omega=1.9)
     with pytest.raises(bgk.BGKError) as e:
