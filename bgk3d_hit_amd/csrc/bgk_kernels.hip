@@ -376,6 +376,10 @@ static hipError_t collide3_diag_sh(const Lattice &L, int src, double omega, doub
     if (zc <= 0 || L.layout != BGK_LAYOUT_PERIODIC) return hipErrorInvalidValue;
     int nblocks;
     Geo2 g = make_geo2(L, T::X, T::Y, zc, &nblocks);
+    const Split3 sp = three_step_split(L, T::X, T::Y, T::ZC);  // as collide3_sh
+    g.full_cols = sp.full_cols;
+    g.zc = sp.zc;
+    nblocks = sp.nblocks;
     g.red = partials;
     g.svisc = svisc;
     const C w = (C)omega, cte1 = (C)(L.shifted ? 1.0 : 0.0);
@@ -401,9 +405,7 @@ static int collide3_diag_parts_t(const Lattice &L) {
     using T = Tile3<S, C>;
     const int zc = three_step_zc(L, T::X, T::Y, T::ZC);
     if (zc <= 0) return 0;
-    int nblocks;
-    (void)make_geo2(L, T::X, T::Y, zc, &nblocks);
-    return nblocks;
+    return three_step_split(L, T::X, T::Y, T::ZC).nblocks;  // collide3_diag_sh's grid
 }
 
 // ------------------------------------------------------------- checksum
