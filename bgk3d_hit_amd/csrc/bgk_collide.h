@@ -981,7 +981,7 @@ __global__ __launch_bounds__(collide3_threads(TX, TY), 1) void k_collide3(Pops<S
                                                                           C omega, C cte1) {
     static_assert(!(RED && (SLAB || FACES)), "RED runs on the periodic lattice alone");
     // the second stage trimmed to what the third reads (s2kind, k3_row)
-    constexpr bool TRIM2 = sizeof(S) == 2 && !FACES && !RED;
+    constexpr bool TRIM2 = (sizeof(S) == 2 || sizeof(C) == 8) && !FACES && !RED;
     constexpr int EX = TX + 4, EY = TY + 4;
     constexpr int NMAIN = TX * EY;
     static_assert(4 * EY <= 64, "the edge wave covers four grown columns");
@@ -1019,9 +1019,9 @@ __global__ __launch_bounds__(collide3_threads(TX, TY), 1) void k_collide3(Pops<S
     // compiler drops their equilibrium and relaxation); the kind is uniform
     // over the second stage's lanes of each wave.
     // Taken for 16-bit storage (the collisions bound the pass; 107 and 124
-    // VGPRs).  FP32 storage spilled registers with it (and trimming the edge
-    // wave alone measured slower); FP64 compute is bound by latency, not by
-    // its collisions.
+    // VGPRs) and FP64 compute (242 VGPRs of its 256; 1.5-4 % faster).  FP32
+    // storage spilled registers with it (and trimming the edge wave alone
+    // measured slower).
     const int s2kind =
         !TRIM2 ? 0 : (tid >= NMAIN ? 3 : (ey == 1 ? 1 : (ey == EY - 2 ? 2 : 0)));
 
